@@ -1,0 +1,416 @@
+"""Host-side mirror of the VapourSynth ``bm3d`` namespace over the HIP C ABI.
+
+The reference plugin exposes seven functions to VapourSynth scripts
+(source/VSPlugin.cpp:430-517)::
+
+    core.bm3d.RGB2OPP(clip input[, int sample=0])
+    core.bm3d.OPP2RGB(clip input[, int sample=0])
+    core.bm3d.Basic(clip input[, clip ref, string profile, float[] sigma, int block_size, ...])
+    core.bm3d.Final(clip input, clip ref[, ...])
+    core.bm3d.VBasic / VFinal(... radius, ps_num, ps_range, ps_step ...)
+    core.bm3d.VAggregate(clip input[, int radius=1, int sample=0])
+
+This module offers the same names, arguments and error messages for clips
+held as numpy arrays (host) or torch tensors on the GPU (device).  A
+:class:`Clip` is evaluated lazily frame by frame, like a VapourSynth node:
+``clip.get_frame(n)`` asks the filter for the input frames it needs
+(``bm3d_frame_range``) and runs ``bm3d_get_frame``.  Every filter runs on
+the HIP library (libbm3d_amd.so); there is no CPU fallback -- if the library
+or a GPU is missing, creating a filter raises :class:`Error`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libbm3d_amd.so")
+
+GRAY, RGB, YUV = 1, 2, 3
+INTEGER, FLOAT = 0, 1
+
+
+class Error(Exception):
+    """Raised where VapourSynth would raise vs.Error."""
+
+
+# --------------------------------------------------------------------------- C ABI
+
+class ClipInfo(C.Structure):
+    _fields_ = [("color_family", C.c_int), ("sample_type", C.c_int), ("bits_per_sample", C.c_int),
+                ("subsampling_w", C.c_int), ("subsampling_h", C.c_int), ("width", C.c_int),
+                ("height", C.c_int), ("num_frames", C.c_int)]
+
+
+class Arg(C.Structure):
+    _fields_ = [("key", C.c_char_p), ("type", C.c_int), ("count", C.c_int),
+                ("ints", C.POINTER(C.c_int64)), ("floats", C.POINTER(C.c_double)), ("data", C.c_char_p)]
+
+
+class Props(C.Structure):
+    _fields_ = [("bm3d_opp", C.c_int64), ("range", C.c_int64), ("matrix", C.c_int64),
+                ("v_radius", C.c_int64), ("v_process", C.c_int64 * 3)]
+
+
+class FrameC(C.Structure):
+    _fields_ = [("data", C.c_void_p * 3), ("stride", C.c_ssize_t * 3), ("width", C.c_int),
+                ("height", C.c_int), ("mem", C.c_int), ("props", Props)]
+
+
+EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
+           "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_debug_matches")
+
+_lib = None
+_initialised = False
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libbm3d_amd.so and declares the C signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise Error(f"bm3d: HIP library not built ({path}); run vapoursynth-bm3d_amd/build.py")
+    L = C.CDLL(path)
+    L.bm3d_init.argtypes = [C.c_int]
+    L.bm3d_last_error.restype = C.c_char_p
+    L.bm3d_create.argtypes = [C.c_char_p, C.POINTER(ClipInfo), C.POINTER(ClipInfo), C.POINTER(Arg), C.c_int,
+                              C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
+    L.bm3d_output_info.argtypes = [C.c_void_p, C.POINTER(ClipInfo)]
+    L.bm3d_frame_range.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.bm3d_get_frame.argtypes = [C.c_void_p, C.c_int, C.POINTER(FrameC), C.POINTER(FrameC), C.POINTER(FrameC)]
+    L.bm3d_free.argtypes = [C.c_void_p]
+    L.bm3d_free.restype = None
+    L.bm3d_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.bm3d_debug_matches.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int]
+    _lib = L
+    return L
+
+
+def init(device: int = 0):
+    global _initialised
+    L = load_library()
+    if not _initialised:
+        if L.bm3d_init(device) != 0:
+            raise Error("bm3d: " + L.bm3d_last_error().decode())
+        _initialised = True
+    return L
+
+
+# --------------------------------------------------------------------------- clips
+
+@dataclass
+class VideoFormat:
+    color_family: int
+    sample_type: int
+    bits_per_sample: int
+    subsampling_w: int = 0
+    subsampling_h: int = 0
+
+    @property
+    def num_planes(self):
+        return 1 if self.color_family == GRAY else 3
+
+    @property
+    def bytes_per_sample(self):
+        return 4 if self.sample_type == FLOAT else (2 if self.bits_per_sample > 8 else 1)
+
+    @property
+    def np_dtype(self):
+        if self.sample_type == FLOAT:
+            return np.float32
+        return np.uint16 if self.bits_per_sample > 8 else np.uint8
+
+
+GRAYS = VideoFormat(GRAY, FLOAT, 32)
+GRAY16 = VideoFormat(GRAY, INTEGER, 16)
+RGBS = VideoFormat(RGB, FLOAT, 32)
+RGB48 = VideoFormat(RGB, INTEGER, 16)
+YUV444PS = VideoFormat(YUV, FLOAT, 32)
+YUV444P16 = VideoFormat(YUV, INTEGER, 16)
+
+
+@dataclass
+class Frame:
+    """One frame: planes are 2-D numpy arrays (host) or torch tensors (GPU)."""
+    planes: List
+    props: Dict[str, object] = field(default_factory=dict)
+
+    @property
+    def on_device(self):
+        return not isinstance(self.planes[0], np.ndarray)
+
+
+class Clip:
+    def __init__(self, fmt: VideoFormat, width: int, height: int, num_frames: int,
+                 getter: Callable[[int], Frame], owner=None):
+        self.format = fmt
+        self.width = width
+        self.height = height
+        self.num_frames = num_frames
+        self._get = getter
+        self._owner = owner
+        self._cache: Dict[int, Frame] = {}
+
+    def get_frame(self, n: int) -> Frame:
+        if n < 0 or n >= self.num_frames:
+            raise Error(f"frame {n} out of range")
+        if n not in self._cache:
+            self._cache[n] = self._get(n)
+        return self._cache[n]
+
+    def frames(self):
+        return [self.get_frame(n) for n in range(self.num_frames)]
+
+    def info(self) -> ClipInfo:
+        f = self.format
+        return ClipInfo(f.color_family, f.sample_type, f.bits_per_sample, f.subsampling_w, f.subsampling_h,
+                        self.width, self.height, self.num_frames)
+
+    @staticmethod
+    def from_frames(fmt: VideoFormat, frames: Sequence[Frame]) -> "Clip":
+        frames = list(frames)
+        p0 = frames[0].planes[0]
+        h, w = p0.shape[-2], p0.shape[-1]
+        return Clip(fmt, w, h, len(frames), lambda n: frames[n])
+
+    @staticmethod
+    def from_arrays(fmt: VideoFormat, arrays, props=None) -> "Clip":
+        """arrays: sequence of (P, H, W) arrays/tensors, one per frame."""
+        frames = [Frame([a[p] for p in range(a.shape[0])], dict(props or {})) for a in arrays]
+        return Clip.from_frames(fmt, frames)
+
+
+# --------------------------------------------------------------------------- plumbing
+
+def _props_to_c(props: Dict[str, object]) -> Props:
+    p = Props(-1, -1, -1, -1, (C.c_int64 * 3)(-1, -1, -1))
+    if "BM3D_OPP" in props:
+        p.bm3d_opp = int(props["BM3D_OPP"])
+    if "_Range" in props:
+        p.range = int(props["_Range"])
+    if "_Matrix" in props:
+        p.matrix = int(props["_Matrix"])
+    if "BM3D_V_radius" in props:
+        p.v_radius = int(props["BM3D_V_radius"])
+    if "BM3D_V_process" in props:
+        v = list(props["BM3D_V_process"])
+        for i in range(min(3, len(v))):
+            p.v_process[i] = int(v[i])
+    return p
+
+
+def _props_from_c(p: Props, base: Dict[str, object]) -> Dict[str, object]:
+    out = {k: v for k, v in base.items()
+           if k not in ("BM3D_OPP", "_Range", "_Matrix", "BM3D_V_radius", "BM3D_V_process")}
+    if p.bm3d_opp >= 0:
+        out["BM3D_OPP"] = p.bm3d_opp
+    if p.range >= 0:
+        out["_Range"] = p.range
+    if p.matrix >= 0:
+        out["_Matrix"] = p.matrix
+    if p.v_radius >= 0:
+        out["BM3D_V_radius"] = p.v_radius
+    if p.v_process[0] >= 0:
+        out["BM3D_V_process"] = [p.v_process[i] for i in range(3)]
+    return out
+
+
+def _plane_ptr(a):
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data, a.strides[0], 0
+    return a.data_ptr(), a.stride(0) * a.element_size(), 1
+
+
+def _frame_to_c(fr: Frame, keep: list) -> FrameC:
+    c = FrameC()
+    mem = None
+    for p, a in enumerate(fr.planes[:3]):
+        if isinstance(a, np.ndarray) and not a.flags.c_contiguous and a.strides[1] != a.itemsize:
+            a = np.ascontiguousarray(a)
+        keep.append(a)
+        ptr, stride, m = _plane_ptr(a)
+        c.data[p] = ptr
+        c.stride[p] = stride
+        mem = m
+    c.width = fr.planes[0].shape[-1]
+    c.height = fr.planes[0].shape[-2]
+    c.mem = mem
+    c.props = _props_to_c(fr.props)
+    return c
+
+
+def _alloc_like(fmt: VideoFormat, w: int, h: int, device):
+    planes = []
+    for p in range(fmt.num_planes):
+        pw = w if p == 0 else w >> fmt.subsampling_w
+        ph = h if p == 0 else h >> fmt.subsampling_h
+        if device is None:
+            planes.append(np.empty((ph, pw), fmt.np_dtype))
+        else:
+            import torch
+            tdt = {np.float32: torch.float32, np.uint16: torch.uint16, np.uint8: torch.uint8}[fmt.np_dtype]
+            planes.append(torch.empty((ph, pw), dtype=tdt, device=device))
+    return planes
+
+
+def _make_args(kwargs: Dict[str, object], keep: list):
+    arr = (Arg * max(1, len(kwargs)))()
+    i = 0
+    for k, v in kwargs.items():
+        if v is None:
+            continue
+        a = Arg()
+        kb = k.encode()
+        keep.append(kb)
+        a.key = kb
+        if isinstance(v, str):
+            db = v.encode()
+            keep.append(db)
+            a.type, a.count, a.data = 2, 1, db
+        else:
+            vals = list(v) if isinstance(v, (list, tuple)) else [v]
+            if any(isinstance(x, float) for x in vals) or k in ("sigma", "th_mse", "hard_thr"):
+                buf = (C.c_double * len(vals))(*[float(x) for x in vals])
+                a.type, a.count, a.floats = 1, len(vals), C.cast(buf, C.POINTER(C.c_double))
+            else:
+                buf = (C.c_int64 * len(vals))(*[int(x) for x in vals])
+                a.type, a.count, a.ints = 0, len(vals), C.cast(buf, C.POINTER(C.c_int64))
+            keep.append(buf)
+        arr[i] = a
+        i += 1
+    return arr, i
+
+
+class _Filter:
+    def __init__(self, name: str, clip: Clip, ref: Optional[Clip], kwargs: Dict[str, object]):
+        L = init()
+        self.L = L
+        self.name = name
+        self.clip = clip
+        self.ref = ref
+        keep: list = []
+        args, nargs = _make_args(kwargs, keep)
+        handle = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        ii = clip.info()
+        ri = ref.info() if ref is not None else None
+        rc = L.bm3d_create(name.encode(), C.byref(ii), C.byref(ri) if ri is not None else None, args, nargs,
+                           C.byref(handle), err, 1024)
+        if rc != 0:
+            msg = err.value.decode() or L.bm3d_last_error().decode()
+            raise Error(msg)
+        self.h = handle
+        info = ClipInfo()
+        L.bm3d_output_info(self.h, C.byref(info))
+        self.out_fmt = VideoFormat(info.color_family, info.sample_type, info.bits_per_sample, info.subsampling_w,
+                                   info.subsampling_h)
+        self.out_w, self.out_h, self.out_n = info.width, info.height, info.num_frames
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.L.bm3d_free(self.h)
+        except Exception:
+            pass
+
+    def get_frame(self, n: int) -> Frame:
+        first, last = C.c_int(), C.c_int()
+        self.L.bm3d_frame_range(self.h, n, C.byref(first), C.byref(last))
+        keep: list = []
+        k = last.value - first.value + 1
+        srcs = (FrameC * k)()
+        refs = (FrameC * k)() if self.ref is not None else None
+        for i, m in enumerate(range(first.value, last.value + 1)):
+            srcs[i] = _frame_to_c(self.clip.get_frame(m), keep)
+            if refs is not None:
+                refs[i] = _frame_to_c(self.ref.get_frame(m), keep)
+        src0 = self.clip.get_frame(n)
+        device = None if not src0.on_device else src0.planes[0].device
+        planes = _alloc_like(self.out_fmt, self.out_w, self.out_h, device)
+        out = Frame(planes, {})
+        dst = _frame_to_c(out, keep)
+        rc = self.L.bm3d_get_frame(self.h, n, srcs, refs, C.byref(dst))
+        if rc != 0:
+            raise Error(f"bm3d.{self.name}: " + self.L.bm3d_last_error().decode())
+        out.props = _props_from_c(dst.props, src0.props)
+        return out
+
+    def stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.L.bm3d_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def matches(self, b: int, cap: int = 256):
+        cnt = C.c_int()
+        ys = (C.c_int * cap)()
+        xs = (C.c_int * cap)()
+        ks = (C.c_float * cap)()
+        rc = self.L.bm3d_debug_matches(self.h, b, C.byref(cnt), ys, xs, ks, cap)
+        if rc != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        n = cnt.value
+        return np.array(ks[:n], np.float32), np.array(ys[:n], np.int32), np.array(xs[:n], np.int32)
+
+
+def _node(name: str, clip: Clip, ref: Optional[Clip], kwargs) -> Clip:
+    f = _Filter(name, clip, ref, kwargs)
+    return Clip(f.out_fmt, f.out_w, f.out_h, f.out_n, f.get_frame, owner=f)
+
+
+# --------------------------------------------------------------------------- the bm3d namespace
+
+def RGB2OPP(input: Clip, sample: Optional[int] = None) -> Clip:
+    return _node("RGB2OPP", input, None, {"sample": sample})
+
+
+def OPP2RGB(input: Clip, sample: Optional[int] = None) -> Clip:
+    return _node("OPP2RGB", input, None, {"sample": sample})
+
+
+def Basic(input: Clip, ref: Optional[Clip] = None, profile: Optional[str] = None, sigma=None,
+          block_size: Optional[int] = None, block_step: Optional[int] = None, group_size: Optional[int] = None,
+          bm_range: Optional[int] = None, bm_step: Optional[int] = None, th_mse: Optional[float] = None,
+          hard_thr: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+    return _node("Basic", input, ref, dict(profile=profile, sigma=sigma, block_size=block_size,
+                                           block_step=block_step, group_size=group_size, bm_range=bm_range,
+                                           bm_step=bm_step, th_mse=th_mse, hard_thr=hard_thr, matrix=matrix))
+
+
+def Final(input: Clip, ref: Clip, profile: Optional[str] = None, sigma=None, block_size: Optional[int] = None,
+          block_step: Optional[int] = None, group_size: Optional[int] = None, bm_range: Optional[int] = None,
+          bm_step: Optional[int] = None, th_mse: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+    return _node("Final", input, ref, dict(profile=profile, sigma=sigma, block_size=block_size,
+                                           block_step=block_step, group_size=group_size, bm_range=bm_range,
+                                           bm_step=bm_step, th_mse=th_mse, matrix=matrix))
+
+
+def VBasic(input: Clip, ref: Optional[Clip] = None, profile: Optional[str] = None, sigma=None,
+           radius: Optional[int] = None, block_size: Optional[int] = None, block_step: Optional[int] = None,
+           group_size: Optional[int] = None, bm_range: Optional[int] = None, bm_step: Optional[int] = None,
+           ps_num: Optional[int] = None, ps_range: Optional[int] = None, ps_step: Optional[int] = None,
+           th_mse: Optional[float] = None, hard_thr: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+    return _node("VBasic", input, ref, dict(profile=profile, sigma=sigma, radius=radius, block_size=block_size,
+                                            block_step=block_step, group_size=group_size, bm_range=bm_range,
+                                            bm_step=bm_step, ps_num=ps_num, ps_range=ps_range, ps_step=ps_step,
+                                            th_mse=th_mse, hard_thr=hard_thr, matrix=matrix))
+
+
+def VFinal(input: Clip, ref: Clip, profile: Optional[str] = None, sigma=None, radius: Optional[int] = None,
+           block_size: Optional[int] = None, block_step: Optional[int] = None, group_size: Optional[int] = None,
+           bm_range: Optional[int] = None, bm_step: Optional[int] = None, ps_num: Optional[int] = None,
+           ps_range: Optional[int] = None, ps_step: Optional[int] = None, th_mse: Optional[float] = None,
+           matrix: Optional[int] = None) -> Clip:
+    return _node("VFinal", input, ref, dict(profile=profile, sigma=sigma, radius=radius, block_size=block_size,
+                                            block_step=block_step, group_size=group_size, bm_range=bm_range,
+                                            bm_step=bm_step, ps_num=ps_num, ps_range=ps_range, ps_step=ps_step,
+                                            th_mse=th_mse, matrix=matrix))
+
+
+def VAggregate(input: Clip, radius: Optional[int] = None, sample: Optional[int] = None) -> Clip:
+    return _node("VAggregate", input, None, {"radius": radius, "sample": sample})

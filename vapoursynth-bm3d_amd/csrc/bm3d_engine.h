@@ -1,0 +1,88 @@
+// Internal declarations shared by the host engine translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "../../include/bm3d_amd.h"
+#include "bm3d_params.h"
+
+namespace bm3d {
+
+int set_error(int code, const std::string& msg);
+void log_warning(const std::string& msg);
+int rcp_bits();
+const float* rcp_dev();
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::bm3d::set_error(BM3D_E_HIP, std::string("HIP error: ") +                  \
+                                                     hipGetErrorString(e_) + " at " #expr);    \
+    } while (0)
+
+enum Fn { kRGB2OPP, kOPP2RGB, kBasic, kFinal, kVBasic, kVFinal, kVAggregate };
+
+// Device allocation that only grows (reused across frames of one filter).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want) {
+        if (want <= bytes) return BM3D_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        if (want == 0) return BM3D_OK;
+        HIP_TRY(hipMalloc(&p, want));
+        bytes = want;
+        return BM3D_OK;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace bm3d
+
+struct bm3d_filter {
+    bm3d::Fn fn;
+    bm3d_clip_info vi{}, rvi{}, ovi{};
+    bool rdef = false;
+    bool wiener = false, temporal = false;
+    // BM3D_Para / VBM3D_Para after argument processing
+    std::string profile = "fast";
+    double sigma[3] = {10, 10, 10};
+    int B = 8, step = 8, G = 8, R = 9, S = 1;
+    double th_mse = 0, lambda = 0;   // th_mse already multiplied by normY
+    int radius = 1, ps_num = 2, ps_range = 4, ps_step = 1;
+    int matrix = 2;
+    int process[3] = {1, 1, 1};
+    int sample = 0;                  // RGB2OPP / OPP2RGB / VAggregate
+    // constant tables
+    bm3d::PlaneTables tables[3]{};
+    bm3d::DevBuf tables_dev, coef_dev, coef_off_dev;
+    // per-frame workspace (reused)
+    bm3d::DevBuf in_raw[2][3 * 33];  // [src/ref][plane * frames] staged host input
+    bm3d::DevBuf fplane[2][3 * 33];  // converted float planes
+    bm3d::DevBuf out_stage[3];
+    bm3d::DevBuf m_count, m_pos, m_key, m_frame, contrib[3], denw[3], ptrs;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    int64_t stat_blocks = 0, stat_members = 0, stat_ssd = 0;
+    int last_nb = 0;
+    ~bm3d_filter() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace bm3d {
+int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst);
+}

@@ -1,0 +1,79 @@
+// Kernel argument blocks and launchers (implemented in bm3d_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "bm3d_params.h"
+
+namespace bm3d {
+
+enum OutMode { kOutFloat = 0, kOutInt = 1, kOutRGB = 2 };
+
+// How the aggregation kernel stores num/den: the float planes as is, a
+// Float2Int per plane (Conversion.hpp:440-452), or FloatYUV2RGB OPP.
+struct OutputSpec {
+    int mode;
+    int bits;          // integer output bits (0 = float)
+    int clip;
+    void* dst[3];
+    size_t pitch[3];   // elements
+    float gain[3], offset[3], lo[3], hi[3];
+    float m[9];        // YUV->RGB coefficients (kOutRGB)
+};
+
+struct FilterArgs {
+    StageParams p;
+    int b0, nb;
+    int wiener;
+    int process[3];
+    const int* count;          // [nb]
+    const int* pos;            // [nb * G] packed (y << 16) | x
+    const int* frame;          // [nb * G] frame index (V-BM3D) or null
+    const float* const* src[3];  // per plane: device array of frame plane pointers
+    const float* const* ref[3];
+    const PlaneTables* tables;   // device array, one per plane
+    const float* rcp_table;    // rcp of 1.m indexed by the top mantissa bits
+    int rcp_shift;             // 23 - (table index bits)
+    const float* coef;         // DCT coefficients
+    const int* coef_off;       // offsets by length N
+    float* contrib[3];         // [nb * G * B * B]
+    float* denw[3];            // [nb]
+};
+
+struct AggArgs {
+    StageParams p;
+    int process[3];
+    int reach;                 // max |member - reference| displacement
+    const int* count;
+    const int* pos;
+    const float* contrib[3];
+    const float* denw[3];
+    OutputSpec out;
+};
+
+struct RgbArgs {
+    const void* src[3];
+    void* dst[3];
+    int src_bits, dst_bits;
+    size_t spitch, dpitch;
+    int W, H;
+    int clip, y_only;
+    float gain[3], offset[3], lo[2], hi[2];
+    float m[9];
+};
+
+size_t match_wave_words(int max_cand, int B, int G);
+size_t match_lds_bytes(int max_cand, int B, int G);
+size_t filter_lds_bytes(int B, int G);
+hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
+                        float* key, hipStream_t st);
+hipError_t launch_filter(const FilterArgs& a, hipStream_t st);
+hipError_t launch_aggregate(const AggArgs& a, hipStream_t st);
+hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
+                           float gain, float offset, int clip, float lo, float hi, hipStream_t st);
+hipError_t launch_rgb2opp(const RgbArgs& a, hipStream_t st);
+hipError_t launch_opp2rgb(const RgbArgs& a, hipStream_t st);
+
+}  // namespace bm3d

@@ -1,0 +1,812 @@
+// HIP kernels for the BM3D / V-BM3D filtering path on CDNA4 (gfx950).
+//
+// Every floating-point operation below is ordered to reproduce the reference
+// plugin's arithmetic exactly (the host build compiles this file with
+// -ffp-contract=off, so a*b+c is never fused unless __builtin_fmaf is
+// written).  Kernels are wave64-native: one wavefront per reference block
+// (matching), per group (transforms) or per 8x8 output tile (aggregation),
+// with cross-lane ballots instead of shared-memory reductions.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bm3d_params.h"
+#include "bm3d_kernels.h"
+
+namespace bm3d {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Position k of the reference-block grid along one axis: 0, s, 2s, ... with
+// the last one clamped to dim - B (source/BM3D_Base.cpp:312-345).
+__device__ __forceinline__ int grid_pos(int k, int step, int last) {
+    int v = k * step;
+    return v > last ? last : v;
+}
+
+// Block::_SearchBoundary (include/Block.h:734-764) in closed form: walk from
+// pos -/+ range towards pos in steps of `step` until inside the plane.
+__device__ __forceinline__ int search_lo(int pos, int range, int step) {
+    int m = pos / step * step;  // largest multiple of step that keeps pos - m >= 0
+    return pos - (m < range ? m : range);
+}
+__device__ __forceinline__ int search_hi(int pos, int last, int range, int step) {
+    int m = (last - pos) / step * step;
+    return pos + (m < range ? m : range);
+}
+
+// SSD of the reference block against the candidate at q, in the SSE2 lane
+// order of Block::BlockMatchingMulti (include/Block.h:575-631): four lane
+// accumulators, per row and 8-column chunk lane k takes column 8c+k and then
+// 8c+4+k; the lanes are folded as ((s0+s1)+s2)+s3; columns beyond the last
+// full chunk are then added one by one in row-major order.  Each operation
+// rounds separately (no FMA).
+template <int B>
+__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch) {
+    constexpr int SW = B - B % 8;
+    float dist = 0.0f;
+    if constexpr (SW > 0) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const float* qr = q + r * pitch;
+#pragma unroll
+            for (int c = 0; c < SW; c += 8) {
+                float d0 = rb[r * B + c + 0] - qr[c + 0];
+                float d1 = rb[r * B + c + 1] - qr[c + 1];
+                float d2 = rb[r * B + c + 2] - qr[c + 2];
+                float d3 = rb[r * B + c + 3] - qr[c + 3];
+                s0 = s0 + d0 * d0;
+                s1 = s1 + d1 * d1;
+                s2 = s2 + d2 * d2;
+                s3 = s3 + d3 * d3;
+                float e0 = rb[r * B + c + 4] - qr[c + 4];
+                float e1 = rb[r * B + c + 5] - qr[c + 5];
+                float e2 = rb[r * B + c + 6] - qr[c + 6];
+                float e3 = rb[r * B + c + 7] - qr[c + 7];
+                s0 = s0 + e0 * e0;
+                s1 = s1 + e1 * e1;
+                s2 = s2 + e2 * e2;
+                s3 = s3 + e3 * e3;
+            }
+        }
+        dist = dist + (((s0 + s1) + s2) + s3);
+    }
+    if constexpr (B - SW > 0) {
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = SW; c < B; ++c) {
+                float t = rb[r * B + c] - q[r * pitch + c];
+                dist = dist + t * t;
+            }
+    }
+    return dist;
+}
+
+// Runtime-B variant (reference block held in LDS).
+__device__ float ssd_generic(const float* rb, const float* __restrict__ q, int pitch, int B) {
+    const int SW = B - B % 8;
+    float dist = 0.0f;
+    if (SW > 0) {
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        for (int r = 0; r < B; ++r) {
+            const float* qr = q + r * pitch;
+            const float* rr = rb + r * B;
+            for (int c = 0; c < SW; c += 8) {
+                float d0 = rr[c + 0] - qr[c + 0];
+                float d1 = rr[c + 1] - qr[c + 1];
+                float d2 = rr[c + 2] - qr[c + 2];
+                float d3 = rr[c + 3] - qr[c + 3];
+                s0 = s0 + d0 * d0;
+                s1 = s1 + d1 * d1;
+                s2 = s2 + d2 * d2;
+                s3 = s3 + d3 * d3;
+                float e0 = rr[c + 4] - qr[c + 4];
+                float e1 = rr[c + 5] - qr[c + 5];
+                float e2 = rr[c + 6] - qr[c + 6];
+                float e3 = rr[c + 7] - qr[c + 7];
+                s0 = s0 + e0 * e0;
+                s1 = s1 + e1 * e1;
+                s2 = s2 + e2 * e2;
+                s3 = s3 + e3 * e3;
+            }
+        }
+        dist = dist + (((s0 + s1) + s2) + s3);
+    }
+    if (B - SW > 0) {
+        for (int r = 0; r < B; ++r)
+            for (int c = SW; c < B; ++c) {
+                float t = rb[r * B + c] - q[r * pitch + c];
+                dist = dist + t * t;
+            }
+    }
+    return dist;
+}
+
+// ---------------------------------------------------------------------------
+// exact top-k selection on a wave
+// ---------------------------------------------------------------------------
+//
+// keys[c] holds the IEEE bit pattern of candidate c's key (positive floats
+// order like their bit patterns) or 0xFFFFFFFF when rejected.  Selects the
+// k smallest by (key, c) -- the order std::stable_sort gives the accepted
+// list, and the order std::partial_sort gives it except among equal keys --
+// and writes them sorted to sel_key / sel_idx.  nc candidates, wave-uniform.
+__device__ void wave_select(const uint32_t* keys, int nc, int naccepted, int k, uint32_t* sel_key,
+                            int* sel_idx) {
+    const int lane = lane_id();
+    uint32_t thr = 0xFFFFFFFEu;  // all accepted keys qualify
+    int need_eq = 0;             // how many keys equal to thr are admitted (when cut)
+    const bool cut = naccepted > k;
+    if (cut) {
+        // radix descent for the k-th smallest key value T:
+        // the largest T with count(key < T) < k
+        uint32_t t = 0;
+        for (int bit = 31; bit >= 0; --bit) {
+            uint32_t trial = t | (1u << bit);
+            int cnt = 0;
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                int c = c0 + lane;
+                bool less = (c < nc) && (keys[c] < trial);
+                cnt += __popcll(__ballot(less));
+            }
+            if (cnt < k) t = trial;
+        }
+        int less_cnt = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            int c = c0 + lane;
+            less_cnt += __popcll(__ballot((c < nc) && (keys[c] < t)));
+        }
+        thr = t;
+        need_eq = k - less_cnt;
+    }
+    // compaction in candidate order
+    int base = 0, eq_seen = 0;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        int c = c0 + lane;
+        uint32_t kv = (c < nc) ? keys[c] : 0xFFFFFFFFu;
+        bool sel;
+        if (cut) {
+            bool eq = kv == thr;
+            uint64_t eqm = __ballot(eq);
+            int eq_rank = eq_seen + __builtin_amdgcn_mbcnt_hi((uint32_t)(eqm >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)eqm, 0));
+            sel = (kv < thr) || (eq && eq_rank < need_eq);
+            eq_seen += __popcll(eqm);
+        } else {
+            sel = kv != 0xFFFFFFFFu;
+        }
+        uint64_t m = __ballot(sel);
+        int pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (sel) {
+            sel_key[pos] = kv;
+            sel_idx[pos] = c;
+        }
+        base += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // rank sort of the (<= 255) selected entries by (key, index)
+    uint32_t rk[4];
+    int ri[4], rr[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int e = lane + 64 * s;
+        rk[s] = e < k ? sel_key[e] : 0xFFFFFFFFu;
+        ri[s] = e < k ? sel_idx[e] : 0x7FFFFFFF;
+        rr[s] = 0;
+    }
+    for (int e2 = 0; e2 < k; ++e2) {
+        uint32_t k2 = sel_key[e2];
+        int i2 = sel_idx[e2];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) rr[s] += (k2 < rk[s]) || (k2 == rk[s] && i2 < ri[s]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int e = lane + 64 * s;
+        if (e < k) {
+            sel_key[k + rr[s]] = rk[s];
+            sel_idx[k + rr[s]] = ri[s];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// K1+K2: full-search block matching, one wave per reference block
+// ---------------------------------------------------------------------------
+//
+// Block::BlockMatchingMulti(range) (include/Block.h:683-729) with
+// BM3D_Process_Base::BlockMatching (source/BM3D_Base.cpp:473-490): the
+// lattice of the search window (row-major, self excluded) is scored, a
+// candidate is accepted iff dist <= thSSE && dist != 0, key =
+// float(double(dist) * distMul), and the G-1 best follow the self entry.
+// Output per block b: count[b], pos[b*G + z] = (y << 16) | x, key[b*G + z].
+
+constexpr int kMatchWaves = 4;
+
+template <int BT>
+__global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restrict__ plane, StageParams p,
+                                                            int b0, int nb, int max_cand, int wave_words,
+                                                            int* __restrict__ out_count,
+                                                            int* __restrict__ out_pos,
+                                                            float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int b = b0 + blockIdx.x * kMatchWaves + wave;
+    if (b >= nb) return;
+    const int B = BT > 0 ? BT : p.block;
+    const int G = p.group;
+    // per-wave LDS: candidate keys, selection scratch (2G keys, 2G indices),
+    // and the reference block when B is not a compile-time constant
+    uint32_t* keys = smem + (size_t)wave * wave_words;
+    uint32_t* sel_key = keys + max_cand;
+    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);
+    float* rbl = reinterpret_cast<float*>(sel_idx + 2 * G);
+
+    const int bj = b / p.nx, bi = b - bj * p.nx;
+    const int j = grid_pos(bj, p.block_step, p.height - B);
+    const int i = grid_pos(bi, p.block_step, p.width - B);
+
+    int* cnt_out = out_count + b;
+    int* pos_out = out_pos + (size_t)b * G;
+    float* key_out = out_key + (size_t)b * G;
+    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
+        // BlockMatching short-cut: the group is the reference block alone
+        if (lane == 0) {
+            *cnt_out = 1;
+            pos_out[0] = (j << 16) | i;
+            key_out[0] = 0.0f;
+        }
+        return;
+    }
+
+    // reference block
+    float rb_reg[BT > 0 ? BT * BT : 1];
+    const float* rb = rbl;
+    if constexpr (BT > 0) {
+#pragma unroll
+        for (int e = 0; e < BT * BT; ++e) rb_reg[e] = plane[(size_t)(j + e / BT) * p.pitch + i + e % BT];
+        rb = rb_reg;
+    } else {
+        for (int e = lane; e < B * B; e += 64) rbl[e] = plane[(size_t)(j + e / B) * p.pitch + i + e % B];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+
+    const int l = search_lo(i, p.range, p.step);
+    const int r = search_hi(i, p.width - B, p.range, p.step);
+    const int t = search_lo(j, p.range, p.step);
+    const int bt = search_hi(j, p.height - B, p.range, p.step);
+    const int ncols = (r - l) / p.step + 1;
+    const int nrows = (bt - t) / p.step + 1;
+    const int nc = ncols * nrows;
+
+    int naccepted = 0;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        int c = c0 + lane;
+        uint32_t kv = 0xFFFFFFFFu;
+        if (c < nc) {
+            int cy = c / ncols;
+            int y = t + cy * p.step;
+            int x = l + (c - cy * ncols) * p.step;
+            if (!(y == j && x == i)) {
+                const float* q = plane + (size_t)y * p.pitch + x;
+                float dist;
+                if constexpr (BT > 0) dist = ssd_fixed<BT>(rb, q, p.pitch);
+                else dist = ssd_generic(rb, q, p.pitch, B);
+                if (dist <= p.th_sse && dist != 0.0f) {
+                    float key = (float)((double)dist * p.dist_mul);
+                    kv = __float_as_uint(key);
+                }
+            }
+            keys[c] = kv;
+        }
+        naccepted += __popcll(__ballot(kv != 0xFFFFFFFFu));
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const int k = naccepted < G - 1 ? naccepted : G - 1;
+    wave_select(keys, nc, naccepted, k, sel_key, sel_idx);
+    if (lane == 0) {
+        *cnt_out = k + 1;
+        pos_out[0] = (j << 16) | i;
+        key_out[0] = 0.0f;
+    }
+    for (int e = lane; e < k; e += 64) {
+        int c = sel_idx[k + e];
+        int cy = c / ncols;
+        int y = t + cy * p.step;
+        int x = l + (c - cy * ncols) * p.step;
+        pos_out[1 + e] = (y << 16) | x;
+        key_out[1 + e] = __uint_as_float(sel_key[k + e]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K3-K7: gather, 3D DCT, shrinkage, inverse 3D DCT, weights
+// ---------------------------------------------------------------------------
+//
+// Transform definition (identical chains in the CPU oracle): unnormalised
+// DCT-II (FFTW REDFT10) forward and DCT-III (REDFT01) inverse along x, then
+// y, then z (inverse: z, y, x), each evaluated with one even/odd split whose
+// sums are fused multiply-add chains in increasing index order.
+// coef(N, k, j) = float(2 cos(pi k (2j+1) / 2N)) lives at coef[off(N) + k*N + j].
+
+__device__ __forceinline__ float cget(const float* __restrict__ coef, const int* __restrict__ coff, int N, int k,
+                                      int j) {
+    return coef[coff[N] + k * N + j];
+}
+
+// Forward 1D DCT-II of a line in LDS (element m at in[m*is]) into out[k*os].
+__device__ void dct2_line(const float* in, int is, float* out, int os, int N, const float* __restrict__ coef,
+                          const int* __restrict__ coff) {
+    const int h = N >> 1;
+    const bool odd = N & 1;
+    for (int k = 0; k < N; ++k) {
+        float acc;
+        if ((k & 1) == 0) {
+            if (h > 0) {
+                acc = cget(coef, coff, N, k, 0) * (in[0] + in[(N - 1) * is]);
+                for (int jj = 1; jj < h; ++jj)
+                    acc = __builtin_fmaf(cget(coef, coff, N, k, jj), in[jj * is] + in[(N - 1 - jj) * is], acc);
+                if (odd) acc = __builtin_fmaf(cget(coef, coff, N, k, h), in[h * is], acc);
+            } else {
+                acc = cget(coef, coff, N, k, 0) * in[0];
+            }
+        } else {
+            acc = cget(coef, coff, N, k, 0) * (in[0] - in[(N - 1) * is]);
+            for (int jj = 1; jj < h; ++jj)
+                acc = __builtin_fmaf(cget(coef, coff, N, k, jj), in[jj * is] - in[(N - 1 - jj) * is], acc);
+        }
+        out[k * os] = acc;
+    }
+}
+
+// Inverse 1D DCT-III of a line in LDS into out.
+__device__ void dct3_line(const float* in, int is, float* out, int os, int N, const float* __restrict__ coef,
+                          const int* __restrict__ coff) {
+    const int h = N >> 1;
+    const bool odd = N & 1;
+    for (int jj = 0; jj < h + (odd ? 1 : 0); ++jj) {
+        float e = in[0];
+        for (int k = 2; k < N; k += 2) e = __builtin_fmaf(cget(coef, coff, N, k, jj), in[k * is], e);
+        float o = 0.0f;
+        if (N > 1 && jj < h) {
+            o = cget(coef, coff, N, 1, jj) * in[is];
+            for (int k = 3; k < N; k += 2) o = __builtin_fmaf(cget(coef, coff, N, k, jj), in[k * is], o);
+        }
+        if (jj < h) {
+            out[jj * os] = e + o;
+            out[(N - 1 - jj) * os] = e - o;
+        } else {
+            out[jj * os] = e;
+        }
+    }
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward (dir=+1) or inverse (dir=-1) separable 3D transform of the group
+// g[z][y][x] (row stride B+1, slice stride B*(B+1)) using scratch t of the
+// same shape.  The result ends up back in g.
+__device__ void transform3d(float* g, float* t, int n, int B, bool forward, const float* __restrict__ coef,
+                            const int* __restrict__ coff) {
+    const int lane = lane_id();
+    const int rs = B + 1, ss = B * (B + 1);
+    if (forward) {
+        for (int L = lane; L < n * B; L += 64) {  // x: lines (z, y)
+            int z = L / B, y = L - z * B;
+            dct2_line(g + z * ss + y * rs, 1, t + z * ss + y * rs, 1, B, coef, coff);
+        }
+        wave_sync_lds();
+        for (int L = lane; L < n * B; L += 64) {  // y: lines (z, x)
+            int z = L / B, x = L - z * B;
+            dct2_line(t + z * ss + x, rs, g + z * ss + x, rs, B, coef, coff);
+        }
+        wave_sync_lds();
+        for (int L = lane; L < B * B; L += 64) {  // z: lines (y, x)
+            int y = L / B, x = L - y * B;
+            dct2_line(g + y * rs + x, ss, t + y * rs + x, ss, n, coef, coff);
+        }
+        wave_sync_lds();
+        for (int e = lane; e < n * ss; e += 64) g[e] = t[e];
+        wave_sync_lds();
+    } else {
+        for (int L = lane; L < B * B; L += 64) {
+            int y = L / B, x = L - y * B;
+            dct3_line(g + y * rs + x, ss, t + y * rs + x, ss, n, coef, coff);
+        }
+        wave_sync_lds();
+        for (int L = lane; L < n * B; L += 64) {
+            int z = L / B, x = L - z * B;
+            dct3_line(t + z * ss + x, rs, g + z * ss + x, rs, B, coef, coff);
+        }
+        wave_sync_lds();
+        for (int L = lane; L < n * B; L += 64) {
+            int z = L / B, y = L - z * B;
+            dct3_line(g + z * ss + y * rs, 1, t + z * ss + y * rs, 1, B, coef, coff);
+        }
+        wave_sync_lds();
+        for (int e = lane; e < n * ss; e += 64) g[e] = t[e];
+        wave_sync_lds();
+    }
+}
+
+// rcp as the host's _mm_rcp_ps computes it: a function of the top mantissa
+// bits scaled by the exponent (table probed on the host at init).
+__device__ __forceinline__ float rcp_host(const float* __restrict__ tbl, int shift, float x) {
+    uint32_t u = __float_as_uint(x);
+    int e = (int)((u >> 23) & 0xff) - 127;
+    float r = __builtin_ldexpf(tbl[(u & 0x7fffff) >> shift], -e);
+    return (u >> 31) ? -r : r;
+}
+
+// Collaborative filtering of one group on one plane (generic B and n):
+// BM3D_Basic_Process::CollaborativeFilter (source/BM3D_Basic.cpp:72-148) and
+// BM3D_Final_Process::CollaborativeFilter (source/BM3D_Final.cpp:50-127).
+// Writes contrib[z][y][x] = float(c * numW) and denw.
+__device__ void filter_group(const float* const* src_planes, const float* const* ref_planes, int pitch,
+                             const int* zyx_pos, int n, int B, bool wiener, const PlaneTables& tb,
+                             const float* __restrict__ rcp_tbl, int rcp_shift, const float* __restrict__ coef,
+                             const int* __restrict__ coff, float* gs, float* gr, float* tmp,
+                             float* __restrict__ contrib, float* __restrict__ denw_out) {
+    const int lane = lane_id();
+    const int rs = B + 1, ss = B * (B + 1);
+    const int BB = B * B;
+    // gather (BlockGroup::From, include/Block.h:1126-1168)
+    for (int e = lane; e < n * BB; e += 64) {
+        int z = e / BB, yx = e - z * BB, y = yx / B, x = yx - y * B;
+        int pz = zyx_pos[3 * z], py = zyx_pos[3 * z + 1], px = zyx_pos[3 * z + 2];
+        gs[z * ss + y * rs + x] = src_planes[pz][(size_t)(py + y) * pitch + px + x];
+        if (wiener) gr[z * ss + y * rs + x] = ref_planes[pz][(size_t)(py + y) * pitch + px + x];
+    }
+    wave_sync_lds();
+    transform3d(gs, tmp, n, B, true, coef, coff);
+    float num_w, den_w;
+    if (!wiener) {
+        // hard threshold: keep iff |c| > thr (BM3D_Basic.cpp:94-134)
+        int kept = 0;
+        for (int e = lane; e < n * BB; e += 64) {
+            int z = e / BB, yx = e - z * BB, y = yx / B, x = yx - y * B;
+            int flag = (x == 0) + (y == 0) + (z == 0);
+            float thr = tb.thr[n - 1][flag];
+            float* c = gs + z * ss + y * rs + x;
+            float s = *c;
+            if (s > thr || s < -thr) ++kept;
+            else *c = 0.0f;
+        }
+        for (int off = 32; off >= 1; off >>= 1) kept += __shfl_xor(kept, off);
+        den_w = kept < 1 ? 1.0f : 1.0f / (float)kept;
+    } else {
+        transform3d(gr, tmp, n, B, true, coef, coff);
+        // Wiener gain (BM3D_Final.cpp:74-112): flat index e of group[z][y][x];
+        // the SSE2 path covers e < size - size%4 with w = r^2 * rcp(r^2 + s^2)
+        // and 4 lane sums of w^2 (lane = e % 4); the tail divides exactly.
+        const int size = n * BB;
+        const int sw = size - size % 4;
+        const float sig2 = tb.wiener_sigma_sqr[n - 1];
+        for (int e = lane; e < size; e += 64) {
+            int z = e / BB, yx = e - z * BB, y = yx / B, x = yx - y * B;
+            int o = z * ss + y * rs + x;
+            float r = gr[o];
+            float r2 = r * r;
+            float w = e < sw ? r2 * rcp_host(rcp_tbl, rcp_shift, r2 + sig2) : r2 / (r2 + sig2);
+            gs[o] = gs[o] * w;
+            tmp[e] = w * w;  // flat order for the ordered sums below
+        }
+        wave_sync_lds();
+        float part = 0.0f;
+        if (lane < 4) {
+            for (int e = lane; e < sw; e += 4) part = part + tmp[e];
+        }
+        float p0 = __shfl(part, 0), p1 = __shfl(part, 1), p2 = __shfl(part, 2), p3 = __shfl(part, 3);
+        float l2 = 0.0f;
+        if (sw > 0) l2 = l2 + (((p0 + p1) + p2) + p3);
+        for (int e = sw; e < size; ++e) l2 = l2 + tmp[e];
+        wave_sync_lds();
+        l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;  // max(FLT_EPSILON, L2)
+        den_w = 1.0f / l2;
+    }
+    transform3d(gs, tmp, n, B, false, coef, coff);
+    num_w = (float)((double)den_w / tb.final_amp[n - 1]);
+    for (int e = lane; e < n * BB; e += 64) {
+        int z = e / BB, yx = e - z * BB, y = yx / B, x = yx - y * B;
+        contrib[e] = gs[z * ss + y * rs + x] * num_w;
+    }
+    if (lane == 0) *denw_out = den_w;
+    wave_sync_lds();
+}
+
+// One wave per group; all processed planes of the group in turn.
+__global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
+    extern __shared__ float fsm[];
+    const int g = a.b0 + blockIdx.x;
+    if (g >= a.nb) return;
+    const int lane = lane_id();
+    const int B = a.p.block, G = a.p.group;
+    const int ss = B * (B + 1);
+    const int n = uniform(a.count[g] < G ? a.count[g] : G);
+    float* gs = fsm;
+    float* gr = gs + G * ss;
+    float* tmp = gr + G * ss;
+    int* zyx = reinterpret_cast<int*>(tmp + G * ss);
+    for (int z = lane; z < n; z += 64) {
+        int pk = a.pos[(size_t)g * G + z];
+        zyx[3 * z] = a.frame ? a.frame[(size_t)g * G + z] : 0;
+        zyx[3 * z + 1] = pk >> 16;
+        zyx[3 * z + 2] = pk & 0xffff;
+    }
+    wave_sync_lds();
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!a.process[pl]) continue;
+        filter_group(a.src[pl], a.ref[pl], a.p.pitch, zyx, n, B, a.wiener, a.tables[pl], a.rcp_table, a.rcp_shift, a.coef,
+                     a.coef_off, gs, gr, tmp, a.contrib[pl] + (size_t)g * G * B * B, a.denw[pl] + g);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K8-K10: ordered gather aggregation, divide and output conversion
+// ---------------------------------------------------------------------------
+//
+// For each output pixel the reference adds float(c * numW) into num and
+// denW into den in (reference-block scan order, group member order)
+// (BlockGroup::AddTo / CountTo, include/Block.h:1239-1259, 1326-1345).  One
+// wave owns an 8x8 tile and walks the reference blocks that can reach it in
+// scan order, so every pixel sees its contributions in exactly that order
+// without atomics; then dst = num / den (BM3D_Base.cpp:455-469).
+
+__device__ __forceinline__ float clipf(float v, float lo, float hi) { return v <= lo ? lo : (v >= hi ? hi : v); }
+
+__device__ __forceinline__ void store_sample(void* base, int bits, size_t idx, float v) {
+    if (bits == 0) static_cast<float*>(base)[idx] = v;
+    else if (bits <= 8) static_cast<uint8_t*>(base)[idx] = (uint8_t)v;
+    else static_cast<uint16_t*>(base)[idx] = (uint16_t)v;
+}
+
+__device__ __forceinline__ float load_sample(const void* base, int bits, size_t idx) {
+    if (bits == 0) return static_cast<const float*>(base)[idx];
+    if (bits <= 8) return (float)static_cast<const uint8_t*>(base)[idx];
+    return (float)static_cast<const uint16_t*>(base)[idx];
+}
+
+// Writes the filtered float triple/single to the destination format.
+__device__ void write_output(const OutputSpec& o, int x, int y, const float* v, const bool* proc) {
+    if (o.mode == kOutFloat) {
+        for (int pl = 0; pl < 3; ++pl)
+            if (proc[pl]) static_cast<float*>(o.dst[pl])[(size_t)y * o.pitch[pl] + x] = v[pl];
+    } else if (o.mode == kOutInt) {
+        for (int pl = 0; pl < 3; ++pl) {
+            if (!proc[pl]) continue;
+            float t = v[pl] * o.gain[pl] + o.offset[pl];
+            t = clipf(t, o.lo[pl], o.hi[pl]);
+            store_sample(o.dst[pl], o.bits, (size_t)y * o.pitch[pl] + x, t);
+        }
+    } else {  // kOutRGB: FloatYUV2RGB OPP (Conversion.hpp:369-392)
+        const float Y = v[0], U = v[1], V = v[2];
+        float r = o.m[0] * Y + o.m[1] * U + o.m[2] * V + o.offset[0];
+        float g = o.m[3] * Y + o.m[5] * V + o.offset[1];
+        float b = o.m[6] * Y + o.m[7] * U + o.m[8] * V + o.offset[2];
+        if (o.clip) {
+            r = clipf(r, o.lo[0], o.hi[0]);
+            g = clipf(g, o.lo[0], o.hi[0]);
+            b = clipf(b, o.lo[0], o.hi[0]);
+        }
+        size_t idx = (size_t)y * o.pitch[0] + x;
+        store_sample(o.dst[0], o.bits, idx, r);
+        store_sample(o.dst[1], o.bits, idx, g);
+        store_sample(o.dst[2], o.bits, idx, b);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_aggregate(AggArgs a) {
+    const int lane = lane_id();
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tiles_x = (a.p.width + 7) >> 3;
+    const int tiles_y = (a.p.height + 7) >> 3;
+    if (tile >= tiles_x * tiles_y) return;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int B = a.p.block, G = a.p.group, BB = B * B;
+    const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+    const bool valid = px < a.p.width && py < a.p.height;
+    const int y0 = ty * 8, x0 = tx * 8;
+    // reference blocks whose search window can reach the tile (conservative)
+    const int lo_y = y0 - B + 1 - a.reach, hi_y = y0 + 7 + a.reach;
+    const int lo_x = x0 - B + 1 - a.reach, hi_x = x0 + 7 + a.reach;
+    const int s = a.p.block_step;
+    int a_lo = lo_y > 0 ? lo_y / s - 1 : 0, a_hi = hi_y / s + 1;
+    int c_lo = lo_x > 0 ? lo_x / s - 1 : 0, c_hi = hi_x / s + 1;
+    a_lo = a_lo < 0 ? 0 : a_lo;
+    c_lo = c_lo < 0 ? 0 : c_lo;
+    a_hi = a_hi > a.p.ny - 1 ? a.p.ny - 1 : a_hi;
+    c_hi = c_hi > a.p.nx - 1 ? a.p.nx - 1 : c_hi;
+    bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
+    float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
+    for (int ar = a_lo; ar <= a_hi; ++ar) {
+        for (int ac = c_lo; ac <= c_hi; ++ac) {
+            const int b = ar * a.p.nx + ac;
+            int n = a.count[b];
+            n = uniform(n < G ? n : G);
+            for (int z0 = 0; z0 < n; z0 += 64) {
+                int z = z0 + lane;
+                int pk = 0;
+                bool ov = false;
+                if (z < n) {
+                    pk = a.pos[(size_t)b * G + z];
+                    int qy = pk >> 16, qx = pk & 0xffff;
+                    ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
+                }
+                uint64_t m = __ballot(ov);
+                while (m) {
+                    int zl = __builtin_ctzll(m);
+                    m &= m - 1;
+                    int q = __builtin_amdgcn_readlane(pk, zl);
+                    int qy = q >> 16, qx = q & 0xffff;
+                    int dy = py - qy, dx = px - qx;
+                    bool cov = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
+                    size_t off = ((size_t)b * G + z0 + zl) * BB + (size_t)(dy * B + dx);
+#pragma unroll
+                    for (int pl = 0; pl < 3; ++pl) {
+                        if (!proc[pl]) continue;
+                        if (cov) {
+                            num[pl] = num[pl] + a.contrib[pl][off];
+                            den[pl] = den[pl] + a.denw[pl][b];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    float v[3];
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) v[pl] = proc[pl] ? num[pl] / den[pl] : 0.0f;
+    write_output(a.out, px, py, v, proc);
+}
+
+// ---------------------------------------------------------------------------
+// input conversions and standalone colour filters
+// ---------------------------------------------------------------------------
+
+// RangeConvert int/float -> float (Conversion.hpp:36-78) for one plane.
+__global__ void k_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
+                           float gain, float offset, int clip, float lo, float hi) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= W || y >= H) return;
+    float v = load_sample(src, bits, (size_t)y * spitch + x);
+    if (bits != 0 || gain != 1.0f || offset != 0.0f || clip) {
+        v = v * gain + offset;
+        if (clip) v = clipf(v, lo, hi);
+    }
+    dst[(size_t)y * dpitch + x] = v;
+}
+
+// MatrixConvert_RGB2YUV / ConvertToY, OPP branch (Conversion.hpp:101-115,
+// 205-236).  ny_only: write Y only.  dst_bits 0 = float output.
+__global__ void k_rgb2opp(RgbArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= a.W || y >= a.H) return;
+    size_t si = (size_t)y * a.spitch + x, di = (size_t)y * a.dpitch + x;
+    float R = load_sample(a.src[0], a.src_bits, si);
+    float G = load_sample(a.src[1], a.src_bits, si);
+    float B = load_sample(a.src[2], a.src_bits, si);
+    float t = ((R + G) + B) * a.gain[0] + a.offset[0];
+    store_sample(a.dst[0], a.dst_bits, di, a.clip ? clipf(t, a.lo[0], a.hi[0]) : t);
+    if (a.y_only) return;
+    t = (R - B) * a.gain[1] + a.offset[1];
+    store_sample(a.dst[1], a.dst_bits, di, a.clip ? clipf(t, a.lo[1], a.hi[1]) : t);
+    t = ((R - G * 2.0f) + B) * a.gain[2] + a.offset[1];
+    store_sample(a.dst[2], a.dst_bits, di, a.clip ? clipf(t, a.lo[1], a.hi[1]) : t);
+}
+
+// MatrixConvert_YUV2RGB, OPP branch (Conversion.hpp:369-392).
+__global__ void k_opp2rgb(RgbArgs a) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (x >= a.W || y >= a.H) return;
+    size_t si = (size_t)y * a.spitch + x, di = (size_t)y * a.dpitch + x;
+    float Y = load_sample(a.src[0], a.src_bits, si);
+    float U = load_sample(a.src[1], a.src_bits, si);
+    float V = load_sample(a.src[2], a.src_bits, si);
+    float r = a.m[0] * Y + a.m[1] * U + a.m[2] * V + a.offset[0];
+    float g = a.m[3] * Y + a.m[5] * V + a.offset[1];
+    float b = a.m[6] * Y + a.m[7] * U + a.m[8] * V + a.offset[2];
+    if (a.clip) {
+        r = clipf(r, a.lo[0], a.hi[0]);
+        g = clipf(g, a.lo[0], a.hi[0]);
+        b = clipf(b, a.lo[0], a.hi[0]);
+    }
+    store_sample(a.dst[0], a.dst_bits, di, r);
+    store_sample(a.dst[1], a.dst_bits, di, g);
+    store_sample(a.dst[2], a.dst_bits, di, b);
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+
+size_t match_wave_words(int max_cand, int B, int G) {
+    return (size_t)max_cand + 4 * (size_t)G + (B == 8 ? 0 : (size_t)B * B);
+}
+
+size_t match_lds_bytes(int max_cand, int B, int G) {
+    return sizeof(uint32_t) * (size_t)kMatchWaves * match_wave_words(max_cand, B, G);
+}
+
+hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
+                        float* key, hipStream_t st) {
+    size_t lds = match_lds_bytes(max_cand, p.block, p.group);
+    int ww = (int)match_wave_words(max_cand, p.block, p.group);
+    int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
+    if (blocks == 0) return hipSuccess;
+    if (p.block == 8)
+        hipLaunchKernelGGL(k_match<8>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, 0, nb, max_cand,
+                           ww, count, pos, key);
+    else
+        hipLaunchKernelGGL(k_match<0>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, 0, nb, max_cand,
+                           ww, count, pos, key);
+    return hipGetLastError();
+}
+
+size_t filter_lds_bytes(int B, int G) {
+    return sizeof(float) * (size_t)3 * G * B * (B + 1) + sizeof(int) * 3 * (size_t)G;
+}
+
+hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
+    if (a.nb == 0) return hipSuccess;
+    size_t lds = filter_lds_bytes(a.p.block, a.p.group);
+    hipLaunchKernelGGL(k_filter, dim3(a.nb), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_aggregate(const AggArgs& a, hipStream_t st) {
+    int tiles = ((a.p.width + 7) / 8) * ((a.p.height + 7) / 8);
+    hipLaunchKernelGGL(k_aggregate, dim3((tiles + 3) / 4), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
+                           float gain, float offset, int clip, float lo, float hi, hipStream_t st) {
+    dim3 grid((W + 255) / 256, H);
+    hipLaunchKernelGGL(k_to_float, grid, dim3(256), 0, st, src, bits, spitch, dst, dpitch, W, H, gain, offset, clip,
+                       lo, hi);
+    return hipGetLastError();
+}
+
+hipError_t launch_rgb2opp(const RgbArgs& a, hipStream_t st) {
+    dim3 grid((a.W + 255) / 256, a.H);
+    hipLaunchKernelGGL(k_rgb2opp, grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_opp2rgb(const RgbArgs& a, hipStream_t st) {
+    dim3 grid((a.W + 255) / 256, a.H);
+    hipLaunchKernelGGL(k_opp2rgb, grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace bm3d

@@ -1,0 +1,461 @@
+// Per-frame dispatch: what VSProcess::process and the process_core_* /
+// Kernel functions of the reference do for one output frame
+// (include/Helper.h:695-725, source/BM3D_Base.cpp:290-754,
+// include/RGB2OPP.h:188-209, include/OPP2RGB.h:188-209), as a sequence of
+// kernel launches on the filter's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bm3d_engine.h"
+#include "bm3d_kernels.h"
+
+namespace bm3d {
+
+namespace {
+
+constexpr int kPitchAlign = 64;  // floats: 256-byte aligned rows
+
+int bytes_of(const bm3d_clip_info& vi) { return vi.sample_type == BM3D_ST_FLOAT ? 4 : (vi.bits_per_sample > 8 ? 2 : 1); }
+int bits_code(const bm3d_clip_info& vi) { return vi.sample_type == BM3D_ST_FLOAT ? 0 : vi.bits_per_sample; }
+int nplanes(const bm3d_clip_info& vi) { return vi.color_family == BM3D_CF_GRAY ? 1 : 3; }
+int plane_w(const bm3d_clip_info& vi, int p) { return p == 0 ? vi.width : vi.width >> vi.subsampling_w; }
+int plane_h(const bm3d_clip_info& vi, int p) { return p == 0 ? vi.height : vi.height >> vi.subsampling_h; }
+
+// GetQuanPara for integer formats (include/Helper.h:285-346).
+void quan_int(int bits, bool full, bool chroma, int& fl, int& ne, int& ce) {
+    if (full) {
+        fl = 0;
+        ne = chroma ? 1 << (bits - 1) : 0;
+        ce = (1 << bits) - 1;
+    } else {
+        fl = 16 << (bits - 8);
+        ne = chroma ? 1 << (bits - 1) : fl;
+        ce = chroma ? 240 << (bits - 8) : 235 << (bits - 8);
+    }
+}
+
+// Int2Float coefficients: RangeConvert<float, int> (include/Conversion.hpp:36-78).
+struct ToFloat {
+    float gain = 1, offset = 0, lo = 0, hi = 1;
+    int clip = 0;
+};
+ToFloat int2float_coefs(int bits, bool full, bool chroma) {
+    ToFloat t;
+    int sf, sn, sc;
+    quan_int(bits, full, chroma, sf, sn, sc);
+    const float df = chroma ? -0.5f : 0.0f, dn = chroma ? 0.0f : df, dc = chroma ? 0.5f : 1.0f;
+    const int srange = sc - sf;
+    const float drange = dc - df;
+    t.clip = (sf < sn && (sf + sc) % 2 == 1) ? 1 : 0;  // PC-range chroma source always clips
+    t.gain = drange / (float)srange;
+    t.offset = -(float)sn * t.gain + dn;
+    t.lo = df;
+    t.hi = dc;
+    return t;
+}
+
+// Float2Int coefficients: RangeConvert<int, float> with clip.
+void float2int_coefs(int bits, bool full, bool chroma, float& gain, float& offset, float& lo, float& hi) {
+    int df, dn, dc;
+    quan_int(bits, full, chroma, df, dn, dc);
+    const float sf = chroma ? -0.5f : 0.0f, sn = chroma ? 0.0f : sf, sc = chroma ? 0.5f : 1.0f;
+    const float srange = sc - sf;
+    const int drange = dc - df;
+    const bool pc = df < dn && (df + dc) % 2 == 1;
+    gain = (float)drange / srange;
+    offset = -sn * gain + (float)dn;
+    offset += pc ? (float)0.499999 : 0.5f;
+    lo = (float)df;
+    hi = (float)dc;
+}
+
+// MatrixConvert_RGB2YUV, OPP branch coefficients (Conversion.hpp:205-236).
+// src_bits 0 = float; dst_bits 0 = float.
+void rgb2opp_coefs(int src_bits, int dst_bits, RgbArgs& a) {
+    float sf = 0.0f, sr;
+    if (src_bits == 0) sr = 1.0f;
+    else sr = (float)((1 << src_bits) - 1);
+    const bool dst_float = dst_bits == 0;
+    float dfy, dcy, dfc, dnc, dcc;
+    bool pc = false;
+    if (dst_float) {
+        dfy = 0.0f; dcy = 1.0f; dfc = -0.5f; dnc = 0.0f; dcc = 0.5f;
+    } else {
+        int a0, b0, c0;
+        quan_int(dst_bits, true, false, a0, b0, c0);
+        dfy = (float)a0; dcy = (float)c0;
+        quan_int(dst_bits, true, true, a0, b0, c0);
+        dfc = (float)a0; dnc = (float)b0; dcc = (float)c0;
+        pc = a0 < b0 && (a0 + c0) % 2 == 1;
+    }
+    const float dry = dcy - dfy, drc = dcc - dfc;
+    a.gain[0] = dry / (sr * 3.0f);
+    a.offset[0] = -sf * 3.0f * a.gain[0] + dfy;
+    if (!dst_float) a.offset[0] += 0.5f;
+    a.gain[1] = drc / (sr * 2.0f);
+    a.gain[2] = drc / (sr * 4.0f);
+    a.offset[1] = dnc;
+    if (!dst_float) a.offset[1] += pc ? (float)0.499999 : 0.5f;
+    a.lo[0] = dfy; a.hi[0] = dcy;
+    a.lo[1] = dfc; a.hi[1] = dcc;
+}
+
+// MatrixConvert_YUV2RGB, OPP branch coefficients (Conversion.hpp:290-417,
+// Specification.h:253-269).  m = Ry Ru Rv Gy Gu Gv By Bu Bv.
+void opp2rgb_coefs(int src_bits, int dst_bits, float m[9], float off[3], float& lo, float& hi) {
+    float sfy, scy, sfc, snc, scc;
+    if (src_bits == 0) {
+        sfy = 0.0f; scy = 1.0f; sfc = -0.5f; snc = 0.0f; scc = 0.5f;
+    } else {
+        int a0, b0, c0;
+        quan_int(src_bits, true, false, a0, b0, c0);
+        sfy = (float)a0; scy = (float)c0;
+        quan_int(src_bits, true, true, a0, b0, c0);
+        sfc = (float)a0; snc = (float)b0; scc = (float)c0;
+    }
+    const bool dst_float = dst_bits == 0;
+    const float dfl = 0.0f, dcl = dst_float ? 1.0f : (float)((1 << dst_bits) - 1);
+    const float sry = scy - sfy, src_ = scc - sfc, dr = dcl - dfl;
+    const float gy = dr / sry, gc = dr / src_;
+    float Ry = 1.0f, Ru = 1.0f, Rv = (float)(2.0L / 3.0L);
+    float Gy = 1.0f, Gu = 0.0f, Gv = (float)(-4.0L / 3.0L);
+    float By = 1.0f, Bu = -1.0f, Bv = (float)(2.0L / 3.0L);
+    Ry *= gy; Ru *= gc; Rv *= gc;
+    Gy *= gy; Gu *= gc; Gv *= gc;
+    By *= gy; Bu *= gc; Bv *= gc;
+    off[0] = -sfy * Ry - snc * (Ru + Rv) + dfl;
+    off[1] = -sfy * Gy - snc * (Gu + Gv) + dfl;
+    off[2] = -sfy * By - snc * (Bu + Bv) + dfl;
+    if (!dst_float) {
+        off[0] += 0.5f;
+        off[1] += 0.5f;
+        off[2] += 0.5f;
+    }
+    m[0] = Ry; m[1] = Ru; m[2] = Rv; m[3] = Gy; m[4] = Gu; m[5] = Gv; m[6] = By; m[7] = Bu; m[8] = Bv;
+    lo = dfl;
+    hi = dcl;
+}
+
+size_t align_pitch(int w) { return (size_t)((w + kPitchAlign - 1) / kPitchAlign) * kPitchAlign; }
+
+// Returns a device pointer + element pitch for plane p of frame fr, staging
+// host memory into buf when needed.
+int device_plane(const bm3d_frame& fr, int p, int w, int h, int bps, DevBuf& buf, hipStream_t st,
+                 const void*& ptr, size_t& pitch_elems) {
+    if (fr.mem == 1) {
+        ptr = fr.data[p];
+        pitch_elems = (size_t)fr.stride[p] / bps;
+        return BM3D_OK;
+    }
+    int rc = buf.ensure((size_t)w * h * bps);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy2DAsync(buf.p, (size_t)w * bps, fr.data[p], fr.stride[p], (size_t)w * bps, h,
+                             hipMemcpyHostToDevice, st));
+    ptr = buf.p;
+    pitch_elems = w;
+    return BM3D_OK;
+}
+
+bool full_range(const bm3d_props& pr) {
+    // NewFrame (include/BM3D_Base.h:189-223): BM3D_OPP=1 or no _Range => full;
+    // otherwise full = (_Range != 0), the inverse of VapourSynth's convention.
+    if (pr.bm3d_opp == 1 || pr.range < 0) return true;
+    return pr.range != 0;
+}
+
+// Copies an unprocessed plane through (the reference leaves it uninitialised).
+int copy_plane(const bm3d_frame& s, bm3d_frame& d, int p, int w, int h, int bps, hipStream_t st) {
+    HIP_TRY(hipMemcpy2DAsync(d.data[p], d.stride[p], s.data[p], s.stride[p], (size_t)w * bps, h, hipMemcpyDefault, st));
+    return BM3D_OK;
+}
+
+int run_color(bm3d_filter* f, const bm3d_frame& s, bm3d_frame& d) {
+    const bm3d_clip_info& vi = f->vi;
+    const int W = vi.width, H = vi.height;
+    const int sb = bytes_of(vi);
+    const int dbits = f->sample ? 0 : 16, db = f->sample ? 4 : 2;
+    RgbArgs a{};
+    int rc;
+    for (int p = 0; p < 3; ++p) {
+        size_t pe;
+        if ((rc = device_plane(s, p, W, H, sb, f->in_raw[0][p], f->stream, a.src[p], pe))) return rc;
+        a.spitch = pe;
+    }
+    const bool dev_out = d.mem == 1;
+    for (int p = 0; p < 3; ++p) {
+        if (dev_out) {
+            a.dst[p] = d.data[p];
+            a.dpitch = (size_t)d.stride[p] / db;
+        } else {
+            if ((rc = f->out_stage[p].ensure((size_t)W * H * db))) return rc;
+            a.dst[p] = f->out_stage[p].p;
+            a.dpitch = W;
+        }
+    }
+    a.src_bits = bits_code(vi);
+    a.dst_bits = dbits;
+    a.W = W;
+    a.H = H;
+    a.clip = dbits != 0;
+    if (f->fn == kRGB2OPP) {
+        rgb2opp_coefs(a.src_bits, dbits, a);
+        HIP_TRY(launch_rgb2opp(a, f->stream));
+        d.props = s.props;
+        d.props.matrix = 2;
+        d.props.bm3d_opp = 1;
+    } else {
+        float lo, hi;
+        opp2rgb_coefs(a.src_bits, dbits, a.m, a.offset, lo, hi);
+        a.lo[0] = lo;
+        a.hi[0] = hi;
+        HIP_TRY(launch_opp2rgb(a, f->stream));
+        d.props = s.props;
+        d.props.matrix = 0;
+        d.props.bm3d_opp = -1;
+    }
+    if (!dev_out)
+        for (int p = 0; p < 3; ++p)
+            HIP_TRY(hipMemcpy2DAsync(d.data[p], d.stride[p], a.dst[p], (size_t)W * db, (size_t)W * db, H,
+                                     hipMemcpyDeviceToHost, f->stream));
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    return BM3D_OK;
+}
+
+// Builds the stage parameters of a Basic/Final filter for a W x H plane.
+StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
+    StageParams p{};
+    p.width = W;
+    p.height = H;
+    p.pitch = (int)pitch;
+    p.block = f->B;
+    p.block_step = f->step;
+    p.group = f->G;
+    p.step = f->S;
+    p.range = f->R / f->S * f->S;
+    auto count = [](int dim, int B, int s) {
+        int last = dim - B, n = 0;
+        for (int j = 0;; j += s) {
+            if (j >= last + s) break;
+            if (j > last) j = last;
+            ++n;
+        }
+        return n;
+    };
+    p.nx = count(W, f->B, f->step);
+    p.ny = count(H, f->B, f->step);
+    // Block::BlockMatchingMulti (include/Block.h:558-560)
+    const double mse2sse = (double)(f->B * f->B) * 1.0f * 1.0f / (double)(255 * 255);
+    p.dist_mul = 1.0 / mse2sse;
+    p.th_sse = (float)(f->th_mse * mse2sse);
+    if (f->G == 1 || f->th_mse <= 0) {
+        p.th_sse = 0.0f;  // kernel takes the short-cut
+        p.dist_mul = 0.0;
+    }
+    p.radius = f->radius;
+    p.ps_num = f->ps_num;
+    p.ps_range = f->ps_range;
+    p.ps_step = f->ps_step;
+    return p;
+}
+
+int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_frame& d) {
+    const bm3d_clip_info& vi = f->vi;
+    const int W = vi.width, H = vi.height;
+    const int bps = bytes_of(vi), bits = bits_code(vi);
+    const bool rgb = vi.color_family == BM3D_CF_RGB;
+    const bool isfloat = vi.sample_type == BM3D_ST_FLOAT;
+    const int P = nplanes(vi);
+    const bool gray_path = vi.color_family == BM3D_CF_GRAY || (vi.color_family == BM3D_CF_YUV && !f->process[1] && !f->process[2]);
+    const bool full = rgb ? true : full_range(s.props);
+    if (s.props.bm3d_opp == 1 && !rgb && f->matrix != 100)
+        log_warning(std::string(f->wiener ? "bm3d.Basic/bm3d.Final" : "bm3d.Basic/bm3d.Final") +
+                    " - warning: There's a frame property \"BM3D_OPP=1\" indicating opponent color space input. "
+                    "You should specify \"matrix=100\" in the filter's argument.");
+    const size_t P_ = align_pitch(W);
+    const hipStream_t st = f->stream;
+    int rc;
+    int proc[3] = {f->process[0], gray_path ? 0 : f->process[1], gray_path ? 0 : f->process[2]};
+    if (P == 1) proc[1] = proc[2] = 0;
+
+    // --- inputs as float planes (process_core_{gray,yuv,rgb})
+    const float* S[3] = {nullptr, nullptr, nullptr};
+    const float* Rf[3] = {nullptr, nullptr, nullptr};
+    const bm3d_frame& rf = f->rdef ? *r : s;
+    if (rgb) {
+        for (int which = 0; which < (f->rdef ? 2 : 1); ++which) {
+            const bm3d_frame& fr = which ? rf : s;
+            RgbArgs a{};
+            for (int p = 0; p < 3; ++p) {
+                size_t pe;
+                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, a.src[p], pe))) return rc;
+                a.spitch = pe;
+                if ((rc = f->fplane[which][p].ensure(P_ * H * sizeof(float)))) return rc;
+                a.dst[p] = f->fplane[which][p].p;
+            }
+            a.dpitch = P_;
+            a.src_bits = bits;
+            a.dst_bits = 0;
+            a.W = W;
+            a.H = H;
+            a.clip = 0;
+            a.y_only = (which == 1 && !f->wiener) ? 1 : 0;  // RGB2FloatY for a Basic ref
+            rgb2opp_coefs(bits, 0, a);
+            HIP_TRY(launch_rgb2opp(a, st));
+            for (int p = 0; p < 3; ++p) (which ? Rf : S)[p] = f->fplane[which][p].as<float>();
+        }
+        if (!f->rdef)
+            for (int p = 0; p < 3; ++p) Rf[p] = S[p];
+    } else {
+        for (int which = 0; which < (f->rdef ? 2 : 1); ++which) {
+            const bm3d_frame& fr = which ? rf : s;
+            for (int p = 0; p < P; ++p) {
+                bool need;
+                if (which == 0) need = proc[p] || (p == 0 && !f->rdef);
+                else need = p == 0 || (f->wiener && proc[p]);
+                if (!need) continue;
+                const void* ptr;
+                size_t pe;
+                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, ptr, pe))) return rc;
+                if ((rc = f->fplane[which][p].ensure(P_ * H * sizeof(float)))) return rc;
+                float* dstp = f->fplane[which][p].as<float>();
+                if (isfloat) {
+                    HIP_TRY(launch_to_float(ptr, 0, pe, dstp, P_, W, H, 1.0f, 0.0f, 0, 0.0f, 1.0f, st));
+                } else {
+                    ToFloat t = int2float_coefs(bits, full, p > 0);
+                    HIP_TRY(launch_to_float(ptr, bits, pe, dstp, P_, W, H, t.gain, t.offset, t.clip, t.lo, t.hi, st));
+                }
+                (which ? Rf : S)[p] = dstp;
+            }
+        }
+        if (!f->rdef)
+            for (int p = 0; p < 3; ++p) Rf[p] = S[p];
+    }
+
+    // --- block matching on the reference luma
+    StageParams sp = stage_params(f, W, H, P_);
+    const int nb = sp.nx * sp.ny;
+    const int G = f->G, B = f->B;
+    const int side = 2 * (sp.range / sp.step) + 1;
+    const int max_cand = side * side;
+    if ((rc = f->m_count.ensure(sizeof(int) * nb))) return rc;
+    if ((rc = f->m_pos.ensure(sizeof(int) * (size_t)nb * G))) return rc;
+    if ((rc = f->m_key.ensure(sizeof(float) * (size_t)nb * G))) return rc;
+    if (match_lds_bytes(max_cand, B, G) > 65536)
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the matching kernel");
+    HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, f->m_count.as<int>(), f->m_pos.as<int>(), f->m_key.as<float>(), st));
+    f->last_nb = nb;
+
+    // --- collaborative filtering
+    FilterArgs fa{};
+    fa.p = sp;
+    fa.b0 = 0;
+    fa.nb = nb;
+    fa.wiener = f->wiener;
+    for (int p = 0; p < 3; ++p) fa.process[p] = proc[p];
+    fa.count = f->m_count.as<int>();
+    fa.pos = f->m_pos.as<int>();
+    fa.frame = nullptr;
+    // device pointer table: [src p0..p2][ref p0..p2]
+    if ((rc = f->ptrs.ensure(sizeof(float*) * 6))) return rc;
+    const float* hp[6] = {S[0], S[1], S[2], Rf[0], Rf[1], Rf[2]};
+    HIP_TRY(hipMemcpyAsync(f->ptrs.p, hp, sizeof(hp), hipMemcpyHostToDevice, st));
+    const float* const* dp = f->ptrs.as<const float* const>();
+    for (int p = 0; p < 3; ++p) {
+        fa.src[p] = dp + p;
+        fa.ref[p] = dp + 3 + p;
+        if (!proc[p]) continue;
+        if ((rc = f->contrib[p].ensure(sizeof(float) * (size_t)nb * G * B * B))) return rc;
+        if ((rc = f->denw[p].ensure(sizeof(float) * nb))) return rc;
+        fa.contrib[p] = f->contrib[p].as<float>();
+        fa.denw[p] = f->denw[p].as<float>();
+    }
+    fa.tables = f->tables_dev.as<PlaneTables>();
+    fa.rcp_table = rcp_dev();
+    fa.rcp_shift = 23 - rcp_bits();
+    fa.coef = f->coef_dev.as<float>();
+    fa.coef_off = f->coef_off_dev.as<int>();
+    if (f->wiener && rcp_bits() == 0) fa.rcp_shift = 23;
+    HIP_TRY(launch_filter(fa, st));
+
+    // --- aggregation + divide + output conversion
+    AggArgs ag{};
+    ag.p = sp;
+    for (int p = 0; p < 3; ++p) {
+        ag.process[p] = proc[p];
+        ag.contrib[p] = fa.contrib[p];
+        ag.denw[p] = fa.denw[p];
+    }
+    ag.reach = sp.range;
+    ag.count = fa.count;
+    ag.pos = fa.pos;
+    OutputSpec& o = ag.out;
+    const bool dev_out = d.mem == 1;
+    const int obits = bits;
+    const int ob = bps;
+    for (int p = 0; p < 3; ++p) {
+        bool used = rgb || proc[p];
+        if (!used) continue;
+        if (dev_out) {
+            o.dst[p] = d.data[p];
+            o.pitch[p] = (size_t)d.stride[p] / ob;
+        } else {
+            if ((rc = f->out_stage[p].ensure((size_t)W * H * ob))) return rc;
+            o.dst[p] = f->out_stage[p].p;
+            o.pitch[p] = W;
+        }
+    }
+    o.bits = obits;
+    if (rgb) {
+        o.mode = kOutRGB;
+        float lo, hi;
+        opp2rgb_coefs(0, obits, o.m, o.offset, lo, hi);
+        o.lo[0] = lo;
+        o.hi[0] = hi;
+        o.clip = obits != 0;
+    } else if (isfloat) {
+        o.mode = kOutFloat;
+    } else {
+        o.mode = kOutInt;
+        o.clip = 1;
+        for (int p = 0; p < 3; ++p) float2int_coefs(obits, full, p > 0, o.gain[p], o.offset[p], o.lo[p], o.hi[p]);
+    }
+    HIP_TRY(launch_aggregate(ag, st));
+
+    // --- outputs
+    for (int p = 0; p < P; ++p) {
+        const int pw = plane_w(vi, p), ph = plane_h(vi, p);
+        bool written = rgb || proc[p];
+        if (written && !dev_out) {
+            HIP_TRY(hipMemcpy2DAsync(d.data[p], d.stride[p], o.dst[p], (size_t)W * ob, (size_t)W * ob, H,
+                                     hipMemcpyDeviceToHost, st));
+        } else if (!written) {
+            if ((rc = copy_plane(s, d, p, pw, ph, bps, st))) return rc;
+        }
+    }
+    d.props = s.props;
+    HIP_TRY(hipStreamSynchronize(st));
+    f->stat_blocks = nb;
+    return BM3D_OK;
+}
+
+}  // namespace
+
+int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst) {
+    (void)n;
+    switch (f->fn) {
+    case kRGB2OPP:
+    case kOPP2RGB:
+        return run_color(f, src[0], *dst);
+    case kBasic:
+    case kFinal:
+        return run_bm3d(f, src[0], ref, *dst);
+    default:
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: function not implemented yet");
+    }
+}
+
+}  // namespace bm3d
