@@ -462,8 +462,7 @@ void bm3d_set_log(bm3d_log_fn fn, void* user) {
 int bm3d_create(const char* function, const bm3d_clip_info* input, const bm3d_clip_info* ref, const bm3d_arg* args,
                 int nargs, bm3d_filter** out, char* err, size_t errlen) {
     if (out) *out = nullptr;
-    int rc = ensure_init();
-    if (rc) return rc;
+    int rc;
     if (!function || !input || !out) return set_error(BM3D_E_ARG, "bm3d_create: null argument");
     static const char* names[] = {"RGB2OPP", "OPP2RGB", "Basic", "Final", "VBasic", "VFinal", "VAggregate"};
     int fi = -1;
@@ -521,6 +520,8 @@ int bm3d_create(const char* function, const bm3d_clip_info* input, const bm3d_cl
     default:
         break;
     }
+    // argument validation needs no device; everything below does
+    if ((rc = ensure_init())) return rc;
     if (f->fn == kBasic || f->fn == kFinal || f->fn == kVBasic || f->fn == kVFinal) {
         // init_filter_data (source/BM3D_Base.cpp:262-283)
         double norms[3];
