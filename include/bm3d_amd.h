@@ -128,10 +128,23 @@ int bm3d_get_frame(bm3d_filter *f, int n, const bm3d_frame *src, const bm3d_fram
 /* Releases the filter (the *_Free callbacks, source/VSPlugin.cpp:80-84 etc.). */
 void bm3d_free(bm3d_filter *f);
 
-/* Number of reference blocks and total selected group members of the last
- * Basic/Final frame (for measurement: algorithmic work of a frame).  No
+/* Work done by the last Basic/Final frame: reference blocks, SSD
+ * evaluations, group members (sum of n) and sum of n^2 over groups -- the
+ * inputs of the algorithmic FLOP count.  No counterpart in the reference. */
+typedef struct bm3d_stats {
+    int64_t ref_blocks;
+    int64_t ssd_evals;
+    int64_t members;
+    int64_t members_sq;
+} bm3d_stats;
+int bm3d_last_stats(const bm3d_filter *f, bm3d_stats *out);
+
+/* Per-kernel timing with HIP events recorded on the filter's own stream
+ * around each launch.  bm3d_profile(f, 1) resets and starts accumulating;
+ * bm3d_profile_read returns entry idx (BM3D_E_ARG past the last one).  No
  * counterpart in the reference. */
-int bm3d_last_stats(const bm3d_filter *f, int64_t *ref_blocks, int64_t *group_members, int64_t *ssd_evals);
+int bm3d_profile(bm3d_filter *f, int enable);
+int bm3d_profile_read(const bm3d_filter *f, int idx, const char **name, double *total_ms, int64_t *launches);
 
 /* Copies the match list of reference block b from the last Basic/Final
  * frame: positions (y, x), keys and count.  For parity tests of block

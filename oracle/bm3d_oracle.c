@@ -632,13 +632,34 @@ int or_vblock_match(const or_filter *f, const float *const *planes, int nframes,
  * Each sum is a chain of fused multiply-adds in increasing index order whose
  * first term is a plain product (forward) or X[0] (inverse even part).  The
  * GPU kernels evaluate the identical chains. */
-static float or_dct_coef(int N, int k, int j)
+static float or_dct_coef_eval(int N, int k, int j)
 {
     return (float)(2.0 * cos(M_PI * (double)k * (double)(2 * j + 1) / (2.0 * (double)N)));
 }
 
+/* Tabulated coefficients for N <= OR_COEF_TAB (same values, evaluated once). */
+#define OR_COEF_TAB 64
+static float *or_coef_tab[OR_COEF_TAB + 1];
+static pthread_once_t or_coef_once = PTHREAD_ONCE_INIT;
+
+static void or_coef_init(void)
+{
+    for (int N = 1; N <= OR_COEF_TAB; ++N) {
+        or_coef_tab[N] = (float *)malloc(sizeof(float) * N * N);
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j) or_coef_tab[N][k * N + j] = or_dct_coef_eval(N, k, j);
+    }
+}
+
+static inline float or_dct_coef(int N, int k, int j)
+{
+    if (N <= OR_COEF_TAB) return or_coef_tab[N][k * N + j];
+    return or_dct_coef_eval(N, k, j);
+}
+
 void or_dct2_1d(const float *x, float *X, int N)
 {
+    pthread_once(&or_coef_once, or_coef_init);
     const int h = N / 2;
     const int odd = N & 1;
     float u[OR_MAX_GROUP], v[OR_MAX_GROUP];
@@ -666,6 +687,7 @@ void or_dct2_1d(const float *x, float *X, int N)
 
 void or_dct3_1d(const float *X, float *x, int N)
 {
+    pthread_once(&or_coef_once, or_coef_init);
     const int h = N / 2;
     const int odd = N & 1;
     for (int j = 0; j < h + odd; ++j) {

@@ -61,8 +61,14 @@ class FrameC(C.Structure):
                 ("height", C.c_int), ("mem", C.c_int), ("props", Props)]
 
 
+class Stats(C.Structure):
+    _fields_ = [("ref_blocks", C.c_int64), ("ssd_evals", C.c_int64), ("members", C.c_int64),
+                ("members_sq", C.c_int64)]
+
+
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
-           "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_debug_matches")
+           "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_profile",
+           "bm3d_profile_read", "bm3d_debug_matches")
 
 _lib = None
 _initialised = False
@@ -85,7 +91,10 @@ def load_library(path: str = LIB_PATH):
     L.bm3d_get_frame.argtypes = [C.c_void_p, C.c_int, C.POINTER(FrameC), C.POINTER(FrameC), C.POINTER(FrameC)]
     L.bm3d_free.argtypes = [C.c_void_p]
     L.bm3d_free.restype = None
-    L.bm3d_last_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.bm3d_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
+    L.bm3d_profile.argtypes = [C.c_void_p, C.c_int]
+    L.bm3d_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int64)]
     L.bm3d_debug_matches.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int]
     _lib = L
@@ -156,6 +165,14 @@ class Clip:
         self._get = getter
         self._owner = owner
         self._cache: Dict[int, Frame] = {}
+
+    @property
+    def filter(self):
+        """The native filter behind this clip (None for source clips)."""
+        return self._owner
+
+    def clear_cache(self):
+        self._cache.clear()
 
     def get_frame(self, n: int) -> Frame:
         if n < 0 or n >= self.num_frames:
@@ -341,10 +358,25 @@ class _Filter:
         out.props = _props_from_c(dst.props, src0.props)
         return out
 
-    def stats(self):
-        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
-        self.L.bm3d_last_stats(self.h, C.byref(a), C.byref(b), C.byref(c))
-        return a.value, b.value, c.value
+    def stats(self) -> Dict[str, int]:
+        s = Stats()
+        if self.L.bm3d_last_stats(self.h, C.byref(s)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return dict(ref_blocks=s.ref_blocks, ssd_evals=s.ssd_evals, members=s.members, members_sq=s.members_sq)
+
+    def profile(self, enable: bool = True):
+        self.L.bm3d_profile(self.h, int(enable))
+
+    def profile_read(self) -> Dict[str, tuple]:
+        out = {}
+        i = 0
+        while True:
+            name, ms, n = C.c_char_p(), C.c_double(), C.c_int64()
+            if self.L.bm3d_profile_read(self.h, i, C.byref(name), C.byref(ms), C.byref(n)) != 0:
+                break
+            out[name.value.decode()] = (ms.value, n.value)
+            i += 1
+        return out
 
     def matches(self, b: int, cap: int = 256):
         cnt = C.c_int()

@@ -554,12 +554,52 @@ int bm3d_frame_range(const bm3d_filter* f, int n, int* first, int* last) {
 
 void bm3d_free(bm3d_filter* f) { delete f; }
 
-int bm3d_last_stats(const bm3d_filter* f, int64_t* ref_blocks, int64_t* group_members, int64_t* ssd_evals) {
-    if (!f) return set_error(BM3D_E_STATE, "bm3d_last_stats: null handle");
-    if (ref_blocks) *ref_blocks = f->stat_blocks;
-    if (group_members) *group_members = f->stat_members;
-    if (ssd_evals) *ssd_evals = f->stat_ssd;
+int bm3d_last_stats(const bm3d_filter* f, bm3d_stats* out) {
+    if (!f || !out) return set_error(BM3D_E_STATE, "bm3d_last_stats: null handle");
+    bm3d_stats s = f->stats;
+    if (f->last_nb > 0 && f->m_count.p && (f->fn == kBasic || f->fn == kFinal)) {
+        std::vector<int> c(f->last_nb);
+        HIP_TRY(hipMemcpy(c.data(), f->m_count.p, sizeof(int) * f->last_nb, hipMemcpyDeviceToHost));
+        s.members = s.members_sq = 0;
+        for (int v : c) {
+            int n = std::min(v, f->G);
+            s.members += n;
+            s.members_sq += (int64_t)n * n;
+        }
+    }
+    *out = s;
     return BM3D_OK;
+}
+
+int bm3d_profile(bm3d_filter* f, int enable) {
+    if (!f) return set_error(BM3D_E_STATE, "bm3d_profile: null handle");
+    std::lock_guard<std::mutex> lk(f->mu);
+    for (int i = 0; i < bm3d_filter::kMaxTimers; ++i) {
+        f->prof_ms[i] = 0;
+        f->prof_n[i] = 0;
+        f->prof_pending[i] = false;
+    }
+    if (enable)
+        for (auto& e : f->prof_ev)
+            if (!e) HIP_TRY(hipEventCreate(&e));
+    f->prof = enable != 0;
+    return BM3D_OK;
+}
+
+int bm3d_profile_read(const bm3d_filter* f, int idx, const char** name, double* total_ms, int64_t* launches) {
+    if (!f) return set_error(BM3D_E_STATE, "bm3d_profile_read: null handle");
+    int k = 0;
+    for (int i = 0; i < bm3d_filter::kMaxTimers; ++i) {
+        if (!f->prof_name[i]) continue;
+        if (k == idx) {
+            if (name) *name = f->prof_name[i];
+            if (total_ms) *total_ms = f->prof_ms[i];
+            if (launches) *launches = f->prof_n[i];
+            return BM3D_OK;
+        }
+        ++k;
+    }
+    return BM3D_E_ARG;
 }
 
 int bm3d_debug_matches(const bm3d_filter* f, int64_t b, int* count, int* ys, int* xs, float* keys, int cap) {
@@ -592,6 +632,33 @@ int bm3d_get_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_fram
 }  // extern "C"
 
 namespace bm3d {
+
+void prof_begin(bm3d_filter* f, int t, const char* name) {
+    if (!f->prof) return;
+    f->prof_name[t] = name;
+    (void)hipEventRecord(f->prof_ev[2 * t], f->stream);
+}
+
+void prof_end(bm3d_filter* f, int t) {
+    if (!f->prof) return;
+    (void)hipEventRecord(f->prof_ev[2 * t + 1], f->stream);
+    f->prof_pending[t] = true;
+}
+
+// Call after the stream has been synchronised.
+void prof_collect(bm3d_filter* f) {
+    if (!f->prof) return;
+    for (int t = 0; t < bm3d_filter::kMaxTimers; ++t) {
+        if (!f->prof_pending[t]) continue;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, f->prof_ev[2 * t], f->prof_ev[2 * t + 1]) == hipSuccess) {
+            f->prof_ms[t] += ms;
+            f->prof_n[t] += 1;
+        }
+        f->prof_pending[t] = false;
+    }
+}
+
 int rcp_bits() { return g_dev.rcp_bits; }
 const float* rcp_dev() { return g_dev.rcp_dev; }
 }  // namespace bm3d

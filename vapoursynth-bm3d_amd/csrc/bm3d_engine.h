@@ -76,13 +76,29 @@ struct bm3d_filter {
     bm3d::DevBuf m_count, m_pos, m_key, m_frame, contrib[3], denw[3], ptrs;
     hipStream_t stream = nullptr;
     std::mutex mu;
-    int64_t stat_blocks = 0, stat_members = 0, stat_ssd = 0;
+    bm3d_stats stats{};
     int last_nb = 0;
+    int last_W = 0, last_H = 0;
+    // per-kernel event timing
+    static constexpr int kMaxTimers = 8;
+    bool prof = false;
+    const char* prof_name[kMaxTimers] = {};
+    double prof_ms[kMaxTimers] = {};
+    int64_t prof_n[kMaxTimers] = {};
+    hipEvent_t prof_ev[2 * kMaxTimers] = {};
+    bool prof_pending[kMaxTimers] = {};
     ~bm3d_filter() {
+        for (auto& e : prof_ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 namespace bm3d {
 int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst);
+// Timer slots used by the launch sequence.
+enum Timer { kTMatch = 0, kTFilter = 1, kTAggregate = 2, kTConvert = 3, kTVAggregate = 4 };
+void prof_begin(bm3d_filter* f, int t, const char* name);
+void prof_end(bm3d_filter* f, int t);
+void prof_collect(bm3d_filter* f);
 }

@@ -203,7 +203,9 @@ int run_color(bm3d_filter* f, const bm3d_frame& s, bm3d_frame& d) {
     a.clip = dbits != 0;
     if (f->fn == kRGB2OPP) {
         rgb2opp_coefs(a.src_bits, dbits, a);
+        prof_begin(f, kTConvert, "k_rgb2opp");
         HIP_TRY(launch_rgb2opp(a, f->stream));
+        prof_end(f, kTConvert);
         d.props = s.props;
         d.props.matrix = 2;
         d.props.bm3d_opp = 1;
@@ -212,7 +214,9 @@ int run_color(bm3d_filter* f, const bm3d_frame& s, bm3d_frame& d) {
         opp2rgb_coefs(a.src_bits, dbits, a.m, a.offset, lo, hi);
         a.lo[0] = lo;
         a.hi[0] = hi;
+        prof_begin(f, kTConvert, "k_opp2rgb");
         HIP_TRY(launch_opp2rgb(a, f->stream));
+        prof_end(f, kTConvert);
         d.props = s.props;
         d.props.matrix = 0;
         d.props.bm3d_opp = -1;
@@ -222,6 +226,7 @@ int run_color(bm3d_filter* f, const bm3d_frame& s, bm3d_frame& d) {
             HIP_TRY(hipMemcpy2DAsync(d.data[p], d.stride[p], a.dst[p], (size_t)W * db, (size_t)W * db, H,
                                      hipMemcpyDeviceToHost, f->stream));
     HIP_TRY(hipStreamSynchronize(f->stream));
+    prof_collect(f);
     return BM3D_OK;
 }
 
@@ -260,6 +265,25 @@ StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
     p.ps_range = f->ps_range;
     p.ps_step = f->ps_step;
     return p;
+}
+
+// Exact number of SSD evaluations of a frame: every lattice position of
+// every reference block's search window except the block itself
+// (include/Block.h:683-707).
+int64_t ssd_count(const StageParams& p) {
+    auto axis = [&](int pos, int last) {
+        int lo = pos - std::min(pos / p.step * p.step, p.range);
+        int hi = pos + std::min((last - pos) / p.step * p.step, p.range);
+        return (int64_t)((hi - lo) / p.step + 1);
+    };
+    std::vector<int64_t> cy(p.ny), cx(p.nx);
+    for (int a = 0; a < p.ny; ++a) cy[a] = axis(std::min(a * p.block_step, p.height - p.block), p.height - p.block);
+    for (int c = 0; c < p.nx; ++c) cx[c] = axis(std::min(c * p.block_step, p.width - p.block), p.width - p.block);
+    int64_t sy = 0, sx = 0;
+    for (auto v : cy) sy += v;
+    for (auto v : cx) sx += v;
+    if (p.th_sse <= 0.0f) return 0;
+    return sy * sx - (int64_t)p.ny * p.nx;
 }
 
 int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_frame& d) {
@@ -346,7 +370,9 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     if ((rc = f->m_key.ensure(sizeof(float) * (size_t)nb * G))) return rc;
     if (match_lds_bytes(max_cand, B, G) > 65536)
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the matching kernel");
+    prof_begin(f, kTMatch, f->wiener ? "k_match(final)" : "k_match(basic)");
     HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, f->m_count.as<int>(), f->m_pos.as<int>(), f->m_key.as<float>(), st));
+    prof_end(f, kTMatch);
     f->last_nb = nb;
 
     // --- collaborative filtering
@@ -379,7 +405,9 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     fa.coef = f->coef_dev.as<float>();
     fa.coef_off = f->coef_off_dev.as<int>();
     if (f->wiener && rcp_bits() == 0) fa.rcp_shift = 23;
+    prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
     HIP_TRY(launch_filter(fa, st));
+    prof_end(f, kTFilter);
 
     // --- aggregation + divide + output conversion
     AggArgs ag{};
@@ -423,7 +451,9 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
         o.clip = 1;
         for (int p = 0; p < 3; ++p) float2int_coefs(obits, full, p > 0, o.gain[p], o.offset[p], o.lo[p], o.hi[p]);
     }
+    prof_begin(f, kTAggregate, f->wiener ? "k_aggregate(final)" : "k_aggregate(basic)");
     HIP_TRY(launch_aggregate(ag, st));
+    prof_end(f, kTAggregate);
 
     // --- outputs
     for (int p = 0; p < P; ++p) {
@@ -438,7 +468,11 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     }
     d.props = s.props;
     HIP_TRY(hipStreamSynchronize(st));
-    f->stat_blocks = nb;
+    prof_collect(f);
+    f->stats.ref_blocks = nb;
+    f->stats.ssd_evals = ssd_count(sp);
+    f->last_W = W;
+    f->last_H = H;
     return BM3D_OK;
 }
 
