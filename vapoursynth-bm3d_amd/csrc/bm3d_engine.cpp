@@ -367,7 +367,6 @@ static std::string parse_bm3d(bm3d_filter* f, const Args& args) {
     else if (!is_valid_matrix((int)mat) || mat != (int)mat) return "Unsupported \"matrix\" specified";
     else f->matrix = (int)mat;
     for (int i = 0; i < 3; ++i) f->process[i] = (vi.color_family != BM3D_CF_RGB && f->sigma[i] == 0) ? 0 : 1;
-    if (vi.color_family == BM3D_CF_GRAY) f->process[1] = f->process[2] = 0;
     if (f->process[1] || f->process[2]) {
         if (vi.subsampling_h || vi.subsampling_w)
             return "input clip: sub-sampled format is not supported when chroma is processed, convert it to YUV444 or RGB first. "

@@ -64,6 +64,48 @@ struct RgbArgs {
     float m[9];
 };
 
+constexpr int kMaxStackFrames = 33;  // 2 * radius + 1 with radius <= 16
+
+struct VMatchArgs {
+    StageParams p;
+    int nb, nframes, cur;
+    int max_cand, ps_cap, cat_cap;
+    const float* planes[kMaxStackFrames];  // reference luma of the window's frames
+    int* count;
+    int* pos;
+    int* frame;
+    float* key;
+};
+
+struct VStackArgs {
+    StageParams p;
+    int nframes, cur;
+    int process[3];
+    const int* count;
+    const int* pos;
+    const int* frame;
+    const float* contrib[3];
+    const float* denw[3];
+    float* dst[3];          // base of the (2r+1)*2 slot stack per plane
+    size_t dst_pitch[3];    // elements
+};
+
+struct VAggArgs {
+    int W, H, nframes, cur, radius;
+    int process[3];
+    const float* stacks[3 * kMaxStackFrames];  // [plane * kMaxStackFrames + neighbour]
+    size_t stack_pitch[3];
+    void* dst[3];
+    size_t dst_pitch[3];
+    int out_bits;
+    float gain[3], offset[3], lo[3], hi[3];
+};
+
+size_t vmatch_lds_bytes(int max_cand, int ps_cap, int cat_cap, int B, int G);
+hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st);
+hipError_t launch_vstack(const VStackArgs& a, hipStream_t st);
+hipError_t launch_vagg(const VAggArgs& a, hipStream_t st);
+
 size_t match_wave_words(int max_cand, int B, int G);
 size_t match_lds_bytes(int max_cand, int B, int G);
 size_t filter_lds_bytes(int B, int G);

@@ -11,220 +11,10 @@
 #include <cstdint>
 
 #include "bm3d_params.h"
+#include "bm3d_device.h"
 #include "bm3d_kernels.h"
 
 namespace bm3d {
-
-// ---------------------------------------------------------------------------
-// small device helpers
-// ---------------------------------------------------------------------------
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Position k of the reference-block grid along one axis: 0, s, 2s, ... with
-// the last one clamped to dim - B (source/BM3D_Base.cpp:312-345).
-__device__ __forceinline__ int grid_pos(int k, int step, int last) {
-    int v = k * step;
-    return v > last ? last : v;
-}
-
-// Block::_SearchBoundary (include/Block.h:734-764) in closed form: walk from
-// pos -/+ range towards pos in steps of `step` until inside the plane.
-__device__ __forceinline__ int search_lo(int pos, int range, int step) {
-    int m = pos / step * step;  // largest multiple of step that keeps pos - m >= 0
-    return pos - (m < range ? m : range);
-}
-__device__ __forceinline__ int search_hi(int pos, int last, int range, int step) {
-    int m = (last - pos) / step * step;
-    return pos + (m < range ? m : range);
-}
-
-// SSD of the reference block against the candidate at q, in the SSE2 lane
-// order of Block::BlockMatchingMulti (include/Block.h:575-631): four lane
-// accumulators, per row and 8-column chunk lane k takes column 8c+k and then
-// 8c+4+k; the lanes are folded as ((s0+s1)+s2)+s3; columns beyond the last
-// full chunk are then added one by one in row-major order.  Each operation
-// rounds separately (no FMA).
-template <int B>
-__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch) {
-    constexpr int SW = B - B % 8;
-    float dist = 0.0f;
-    if constexpr (SW > 0) {
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const float* qr = q + r * pitch;
-#pragma unroll
-            for (int c = 0; c < SW; c += 8) {
-                float d0 = rb[r * B + c + 0] - qr[c + 0];
-                float d1 = rb[r * B + c + 1] - qr[c + 1];
-                float d2 = rb[r * B + c + 2] - qr[c + 2];
-                float d3 = rb[r * B + c + 3] - qr[c + 3];
-                s0 = s0 + d0 * d0;
-                s1 = s1 + d1 * d1;
-                s2 = s2 + d2 * d2;
-                s3 = s3 + d3 * d3;
-                float e0 = rb[r * B + c + 4] - qr[c + 4];
-                float e1 = rb[r * B + c + 5] - qr[c + 5];
-                float e2 = rb[r * B + c + 6] - qr[c + 6];
-                float e3 = rb[r * B + c + 7] - qr[c + 7];
-                s0 = s0 + e0 * e0;
-                s1 = s1 + e1 * e1;
-                s2 = s2 + e2 * e2;
-                s3 = s3 + e3 * e3;
-            }
-        }
-        dist = dist + (((s0 + s1) + s2) + s3);
-    }
-    if constexpr (B - SW > 0) {
-#pragma unroll
-        for (int r = 0; r < B; ++r)
-#pragma unroll
-            for (int c = SW; c < B; ++c) {
-                float t = rb[r * B + c] - q[r * pitch + c];
-                dist = dist + t * t;
-            }
-    }
-    return dist;
-}
-
-// Runtime-B variant (reference block held in LDS).
-__device__ float ssd_generic(const float* rb, const float* __restrict__ q, int pitch, int B) {
-    const int SW = B - B % 8;
-    float dist = 0.0f;
-    if (SW > 0) {
-        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-        for (int r = 0; r < B; ++r) {
-            const float* qr = q + r * pitch;
-            const float* rr = rb + r * B;
-            for (int c = 0; c < SW; c += 8) {
-                float d0 = rr[c + 0] - qr[c + 0];
-                float d1 = rr[c + 1] - qr[c + 1];
-                float d2 = rr[c + 2] - qr[c + 2];
-                float d3 = rr[c + 3] - qr[c + 3];
-                s0 = s0 + d0 * d0;
-                s1 = s1 + d1 * d1;
-                s2 = s2 + d2 * d2;
-                s3 = s3 + d3 * d3;
-                float e0 = rr[c + 4] - qr[c + 4];
-                float e1 = rr[c + 5] - qr[c + 5];
-                float e2 = rr[c + 6] - qr[c + 6];
-                float e3 = rr[c + 7] - qr[c + 7];
-                s0 = s0 + e0 * e0;
-                s1 = s1 + e1 * e1;
-                s2 = s2 + e2 * e2;
-                s3 = s3 + e3 * e3;
-            }
-        }
-        dist = dist + (((s0 + s1) + s2) + s3);
-    }
-    if (B - SW > 0) {
-        for (int r = 0; r < B; ++r)
-            for (int c = SW; c < B; ++c) {
-                float t = rb[r * B + c] - q[r * pitch + c];
-                dist = dist + t * t;
-            }
-    }
-    return dist;
-}
-
-// ---------------------------------------------------------------------------
-// exact top-k selection on a wave
-// ---------------------------------------------------------------------------
-//
-// keys[c] holds the IEEE bit pattern of candidate c's key (positive floats
-// order like their bit patterns) or 0xFFFFFFFF when rejected.  Selects the
-// k smallest by (key, c) -- the order std::stable_sort gives the accepted
-// list, and the order std::partial_sort gives it except among equal keys --
-// and writes them sorted to sel_key / sel_idx.  nc candidates, wave-uniform.
-__device__ void wave_select(const uint32_t* keys, int nc, int naccepted, int k, uint32_t* sel_key,
-                            int* sel_idx) {
-    const int lane = lane_id();
-    uint32_t thr = 0xFFFFFFFEu;  // all accepted keys qualify
-    int need_eq = 0;             // how many keys equal to thr are admitted (when cut)
-    const bool cut = naccepted > k;
-    if (cut) {
-        // radix descent for the k-th smallest key value T:
-        // the largest T with count(key < T) < k
-        uint32_t t = 0;
-        for (int bit = 31; bit >= 0; --bit) {
-            uint32_t trial = t | (1u << bit);
-            int cnt = 0;
-            for (int c0 = 0; c0 < nc; c0 += 64) {
-                int c = c0 + lane;
-                bool less = (c < nc) && (keys[c] < trial);
-                cnt += __popcll(__ballot(less));
-            }
-            if (cnt < k) t = trial;
-        }
-        int less_cnt = 0;
-        for (int c0 = 0; c0 < nc; c0 += 64) {
-            int c = c0 + lane;
-            less_cnt += __popcll(__ballot((c < nc) && (keys[c] < t)));
-        }
-        thr = t;
-        need_eq = k - less_cnt;
-    }
-    // compaction in candidate order
-    int base = 0, eq_seen = 0;
-    for (int c0 = 0; c0 < nc; c0 += 64) {
-        int c = c0 + lane;
-        uint32_t kv = (c < nc) ? keys[c] : 0xFFFFFFFFu;
-        bool sel;
-        if (cut) {
-            bool eq = kv == thr;
-            uint64_t eqm = __ballot(eq);
-            int eq_rank = eq_seen + __builtin_amdgcn_mbcnt_hi((uint32_t)(eqm >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)eqm, 0));
-            sel = (kv < thr) || (eq && eq_rank < need_eq);
-            eq_seen += __popcll(eqm);
-        } else {
-            sel = kv != 0xFFFFFFFFu;
-        }
-        uint64_t m = __ballot(sel);
-        int pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        if (sel) {
-            sel_key[pos] = kv;
-            sel_idx[pos] = c;
-        }
-        base += __popcll(m);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // rank sort of the (<= 255) selected entries by (key, index)
-    uint32_t rk[4];
-    int ri[4], rr[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int e = lane + 64 * s;
-        rk[s] = e < k ? sel_key[e] : 0xFFFFFFFFu;
-        ri[s] = e < k ? sel_idx[e] : 0x7FFFFFFF;
-        rr[s] = 0;
-    }
-    for (int e2 = 0; e2 < k; ++e2) {
-        uint32_t k2 = sel_key[e2];
-        int i2 = sel_idx[e2];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) rr[s] += (k2 < rk[s]) || (k2 == rk[s] && i2 < ri[s]);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int e = lane + 64 * s;
-        if (e < k) {
-            sel_key[k + rr[s]] = rk[s];
-            sel_idx[k + rr[s]] = ri[s];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------------------
 // K1+K2: full-search block matching, one wave per reference block
@@ -403,12 +193,6 @@ __device__ void dct3_line(const float* in, int is, float* out, int os, int N, co
     }
 }
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Forward (dir=+1) or inverse (dir=-1) separable 3D transform of the group
 // g[z][y][x] (row stride B+1, slice stride B*(B+1)) using scratch t of the
 // same shape.  The result ends up back in g.
@@ -453,15 +237,6 @@ __device__ void transform3d(float* g, float* t, int n, int B, bool forward, cons
         for (int e = lane; e < n * ss; e += 64) g[e] = t[e];
         wave_sync_lds();
     }
-}
-
-// rcp as the host's _mm_rcp_ps computes it: a function of the top mantissa
-// bits scaled by the exponent (table probed on the host at init).
-__device__ __forceinline__ float rcp_host(const float* __restrict__ tbl, int shift, float x) {
-    uint32_t u = __float_as_uint(x);
-    int e = (int)((u >> 23) & 0xff) - 127;
-    float r = __builtin_ldexpf(tbl[(u & 0x7fffff) >> shift], -e);
-    return (u >> 31) ? -r : r;
 }
 
 // Collaborative filtering of one group on one plane (generic B and n):
@@ -577,20 +352,6 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
 // wave owns an 8x8 tile and walks the reference blocks that can reach it in
 // scan order, so every pixel sees its contributions in exactly that order
 // without atomics; then dst = num / den (BM3D_Base.cpp:455-469).
-
-__device__ __forceinline__ float clipf(float v, float lo, float hi) { return v <= lo ? lo : (v >= hi ? hi : v); }
-
-__device__ __forceinline__ void store_sample(void* base, int bits, size_t idx, float v) {
-    if (bits == 0) static_cast<float*>(base)[idx] = v;
-    else if (bits <= 8) static_cast<uint8_t*>(base)[idx] = (uint8_t)v;
-    else static_cast<uint16_t*>(base)[idx] = (uint16_t)v;
-}
-
-__device__ __forceinline__ float load_sample(const void* base, int bits, size_t idx) {
-    if (bits == 0) return static_cast<const float*>(base)[idx];
-    if (bits <= 8) return (float)static_cast<const uint8_t*>(base)[idx];
-    return (float)static_cast<const uint16_t*>(base)[idx];
-}
 
 // Writes the filtered float triple/single to the destination format.
 __device__ void write_output(const OutputSpec& o, int x, int y, const float* v, const bool* proc) {

@@ -304,6 +304,15 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     int rc;
     int proc[3] = {f->process[0], gray_path ? 0 : f->process[1], gray_path ? 0 : f->process[2]};
     if (P == 1) proc[1] = proc[2] = 0;
+    if (!rgb && !proc[0] && !proc[1] && !proc[2]) {
+        // VSProcess::process returns the source frame untouched when no plane
+        // is processed (include/Helper.h:697-700)
+        for (int p = 0; p < P; ++p)
+            if ((rc = copy_plane(s, d, p, plane_w(vi, p), plane_h(vi, p), bps, st))) return rc;
+        d.props = s.props;
+        HIP_TRY(hipStreamSynchronize(st));
+        return BM3D_OK;
+    }
 
     // --- inputs as float planes (process_core_{gray,yuv,rgb})
     const float* S[3] = {nullptr, nullptr, nullptr};
@@ -476,6 +485,288 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     return BM3D_OK;
 }
 
+// Converts the planes of one input frame to float the way process_core_*
+// does (source/BM3D_Base.cpp:497-754, source/VBM3D_Base.cpp:666-1039).
+// which: 0 = src, 1 = ref.  slot: frame index inside the window.  need[p]
+// tells which planes are required.  Results go to out[p].
+int convert_frame(bm3d_filter* f, const bm3d_frame& fr, int which, int slot, const bool need[3], bool full,
+                  size_t P_, const float* out[3]) {
+    const bm3d_clip_info& vi = f->vi;
+    const int W = vi.width, H = vi.height;
+    const int bps = bytes_of(vi), bits = bits_code(vi);
+    const hipStream_t st = f->stream;
+    int rc;
+    if (vi.color_family == BM3D_CF_RGB) {
+        RgbArgs a{};
+        for (int p = 0; p < 3; ++p) {
+            size_t pe;
+            if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][3 * slot + p], st, a.src[p], pe))) return rc;
+            a.spitch = pe;
+            if ((rc = f->fplane[which][3 * slot + p].ensure(P_ * H * sizeof(float)))) return rc;
+            a.dst[p] = f->fplane[which][3 * slot + p].p;
+            out[p] = f->fplane[which][3 * slot + p].as<float>();
+        }
+        a.dpitch = P_;
+        a.src_bits = bits;
+        a.dst_bits = 0;
+        a.W = W;
+        a.H = H;
+        a.clip = 0;
+        a.y_only = (need[0] && !need[1] && !need[2]) ? 1 : 0;
+        rgb2opp_coefs(bits, 0, a);
+        HIP_TRY(launch_rgb2opp(a, st));
+        return BM3D_OK;
+    }
+    for (int p = 0; p < nplanes(vi); ++p) {
+        if (!need[p]) continue;
+        const void* ptr;
+        size_t pe;
+        if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][3 * slot + p], st, ptr, pe))) return rc;
+        if ((rc = f->fplane[which][3 * slot + p].ensure(P_ * H * sizeof(float)))) return rc;
+        float* dstp = f->fplane[which][3 * slot + p].as<float>();
+        if (bits == 0) {
+            HIP_TRY(launch_to_float(ptr, 0, pe, dstp, P_, W, H, 1.0f, 0.0f, 0, 0.0f, 1.0f, st));
+        } else {
+            ToFloat t = int2float_coefs(bits, full, p > 0);
+            HIP_TRY(launch_to_float(ptr, bits, pe, dstp, P_, W, H, t.gain, t.offset, t.clip, t.lo, t.hi, st));
+        }
+        out[p] = dstp;
+    }
+    return BM3D_OK;
+}
+
+// VBM3D_Process_Base::process_core_* + Kernel (source/VBM3D_Base.cpp:410-1039):
+// the output frame is the float num/den stack of the window's frames.
+int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame& d) {
+    const bm3d_clip_info& vi = f->vi;
+    const int W = vi.width, H = vi.height;
+    const bool rgb = vi.color_family == BM3D_CF_RGB;
+    const int P = nplanes(vi);
+    int first, last;
+    bm3d_frame_range(f, n, &first, &last);
+    const int T = last - first + 1, cur = n - first;
+    if (T > kMaxStackFrames) return set_error(BM3D_E_UNSUPPORTED, "bm3d: temporal window too large");
+    const bm3d_frame& s = src[cur];
+    const bool gray_path = vi.color_family == BM3D_CF_GRAY || (vi.color_family == BM3D_CF_YUV && !f->process[1] && !f->process[2]);
+    const bool full = rgb ? true : full_range(s.props);
+    if (s.props.bm3d_opp == 1 && !rgb && f->matrix != 100)
+        log_warning("bm3d.VBasic/bm3d.VFinal - warning: There's a frame property \"BM3D_OPP=1\" indicating opponent "
+                    "color space input. You should specify \"matrix=100\" in the filter's argument.");
+    int proc[3] = {f->process[0], gray_path ? 0 : f->process[1], gray_path ? 0 : f->process[2]};
+    if (P == 1) proc[1] = proc[2] = 0;
+    if (rgb) proc[0] = proc[1] = proc[2] = 1;
+    const size_t P_ = align_pitch(W);
+    const hipStream_t st = f->stream;
+    int rc;
+    // --- float planes of every frame of the window
+    std::vector<const float*> S(3 * T, nullptr), Rf(3 * T, nullptr);
+    for (int t = 0; t < T; ++t) {
+        bool need_s[3], need_r[3];
+        for (int p = 0; p < 3; ++p) {
+            need_s[p] = p < P && (proc[p] || (p == 0 && !f->rdef));
+            need_r[p] = p < P && (p == 0 || (f->wiener && proc[p]));
+        }
+        if (rgb) need_s[0] = need_s[1] = need_s[2] = true;
+        const float* so[3] = {nullptr, nullptr, nullptr};
+        if ((rc = convert_frame(f, src[t], 0, t, need_s, full, P_, so))) return rc;
+        for (int p = 0; p < 3; ++p) S[3 * t + p] = so[p];
+        if (f->rdef) {
+            const float* ro[3] = {nullptr, nullptr, nullptr};
+            if (rgb && !f->wiener) need_r[1] = need_r[2] = false;
+            if ((rc = convert_frame(f, ref[t], 1, t, need_r, full, P_, ro))) return rc;
+            for (int p = 0; p < 3; ++p) Rf[3 * t + p] = ro[p];
+        } else {
+            for (int p = 0; p < 3; ++p) Rf[3 * t + p] = so[p];
+        }
+    }
+    // --- matching
+    StageParams sp = stage_params(f, W, H, P_);
+    const int nb = sp.nx * sp.ny, G = f->G, B = f->B;
+    const int side = 2 * (sp.range / sp.step) + 1;
+    const int ps_range = f->ps_range / f->ps_step * f->ps_step;
+    const int ps_side = 2 * (ps_range / f->ps_step) + 1;
+    VMatchArgs va{};
+    va.p = sp;
+    va.nb = nb;
+    va.nframes = T;
+    va.cur = cur;
+    va.max_cand = side * side;
+    va.ps_cap = ps_side * ps_side * std::max(1, std::min(f->ps_num, G));
+    va.cat_cap = G * T;
+    if (vmatch_lds_bytes(va.max_cand, va.ps_cap, va.cat_cap, B, G) > 65536)
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the V-BM3D matching kernel");
+    for (int t = 0; t < T; ++t) va.planes[t] = Rf[3 * t];
+    if ((rc = f->m_count.ensure(sizeof(int) * nb))) return rc;
+    if ((rc = f->m_pos.ensure(sizeof(int) * (size_t)nb * G))) return rc;
+    if ((rc = f->m_key.ensure(sizeof(float) * (size_t)nb * G))) return rc;
+    if ((rc = f->m_frame.ensure(sizeof(int) * (size_t)nb * G))) return rc;
+    va.count = f->m_count.as<int>();
+    va.pos = f->m_pos.as<int>();
+    va.frame = f->m_frame.as<int>();
+    va.key = f->m_key.as<float>();
+    prof_begin(f, kTMatch, f->wiener ? "k_vmatch(final)" : "k_vmatch(basic)");
+    HIP_TRY(launch_vmatch(va, st));
+    prof_end(f, kTMatch);
+    f->last_nb = nb;
+    // --- collaborative filtering over frames
+    FilterArgs fa{};
+    fa.p = sp;
+    fa.nb = nb;
+    fa.wiener = f->wiener;
+    for (int p = 0; p < 3; ++p) fa.process[p] = proc[p];
+    fa.count = va.count;
+    fa.pos = va.pos;
+    fa.frame = va.frame;
+    if ((rc = f->ptrs.ensure(sizeof(float*) * 6 * T))) return rc;
+    std::vector<const float*> hp(6 * T);
+    for (int p = 0; p < 3; ++p)
+        for (int t = 0; t < T; ++t) {
+            hp[p * T + t] = S[3 * t + p];
+            hp[3 * T + p * T + t] = Rf[3 * t + p];
+        }
+    HIP_TRY(hipMemcpyAsync(f->ptrs.p, hp.data(), sizeof(float*) * 6 * T, hipMemcpyHostToDevice, st));
+    const float* const* dp = f->ptrs.as<const float* const>();
+    for (int p = 0; p < 3; ++p) {
+        fa.src[p] = dp + p * T;
+        fa.ref[p] = dp + 3 * T + p * T;
+        if (!proc[p]) continue;
+        if ((rc = f->contrib[p].ensure(sizeof(float) * (size_t)nb * G * B * B))) return rc;
+        if ((rc = f->denw[p].ensure(sizeof(float) * nb))) return rc;
+        fa.contrib[p] = f->contrib[p].as<float>();
+        fa.denw[p] = f->denw[p].as<float>();
+    }
+    fa.tables = f->tables_dev.as<PlaneTables>();
+    fa.rcp_table = rcp_dev();
+    fa.rcp_shift = rcp_bits() == 0 ? 23 : 23 - rcp_bits();
+    fa.coef = f->coef_dev.as<float>();
+    fa.coef_off = f->coef_off_dev.as<int>();
+    prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
+    HIP_TRY(launch_filter(fa, st));
+    prof_end(f, kTFilter);
+    // --- aggregation into the stack
+    const int slots = (2 * f->radius + 1) * 2;
+    const bool dev_out = d.mem == 1;
+    VStackArgs vs{};
+    vs.p = sp;
+    vs.nframes = T;
+    vs.cur = cur;
+    vs.count = va.count;
+    vs.pos = va.pos;
+    vs.frame = va.frame;
+    for (int p = 0; p < 3; ++p) {
+        vs.process[p] = proc[p];
+        vs.contrib[p] = fa.contrib[p];
+        vs.denw[p] = fa.denw[p];
+        if (!proc[p]) continue;
+        if (dev_out) {
+            vs.dst[p] = static_cast<float*>(d.data[p]);
+            vs.dst_pitch[p] = (size_t)d.stride[p] / sizeof(float);
+        } else {
+            if ((rc = f->out_stage[p].ensure((size_t)W * H * slots * sizeof(float)))) return rc;
+            vs.dst[p] = f->out_stage[p].as<float>();
+            vs.dst_pitch[p] = W;
+        }
+    }
+    prof_begin(f, kTAggregate, f->wiener ? "k_vstack(final)" : "k_vstack(basic)");
+    HIP_TRY(launch_vstack(vs, st));
+    prof_end(f, kTAggregate);
+    if (!dev_out) {
+        const int s0 = 2 * (f->radius - cur);
+        for (int p = 0; p < 3; ++p) {
+            if (!proc[p]) continue;
+            char* hd = static_cast<char*>(d.data[p]) + (size_t)s0 * H * d.stride[p];
+            const float* sd = vs.dst[p] + (size_t)s0 * H * W;
+            HIP_TRY(hipMemcpy2DAsync(hd, d.stride[p], sd, (size_t)W * sizeof(float), (size_t)W * sizeof(float),
+                                     (size_t)2 * T * H, hipMemcpyDeviceToHost, st));
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    prof_collect(f);
+    f->stats.ref_blocks = nb;
+    d.props = s.props;
+    if (rgb) d.props.bm3d_opp = 1;
+    d.props.v_radius = f->radius;
+    for (int p = 0; p < 3; ++p) d.props.v_process[p] = f->process[p];
+    return BM3D_OK;
+}
+
+// VAggregate_Process (source/VAggregate.cpp:93-334, include/VAggregate.h:98-231).
+int run_vaggregate(bm3d_filter* f, int n, const bm3d_frame* src, bm3d_frame& d) {
+    const bm3d_clip_info& vi = f->vi;
+    const int W = vi.width;
+    const int r = f->radius;
+    const int H = vi.height / (r * 2 + 1) / 2;
+    const int P = nplanes(vi);
+    int first, last;
+    bm3d_frame_range(f, n, &first, &last);
+    const int T = last - first + 1, cur = n - first;
+    if (T > kMaxStackFrames) return set_error(BM3D_E_UNSUPPORTED, "bm3d: temporal window too large");
+    const bm3d_frame& s = src[cur];
+    if (s.props.v_radius < 0)
+        log_warning("bm3d.VAggregate - warning: No frame property \"BM3D_V_radius\" exists in the input frame. "
+                    "Make sure you call bm3d.VAggregate next to bm3d.VBasic or bm3d.VFinal.");
+    else if (s.props.v_radius != r)
+        log_warning("bm3d.VAggregate - warning: Mismatch between argument \"radius=" + std::to_string(r) +
+                    "\" and the input frame property \"BM3D_V_radius=" + std::to_string(s.props.v_radius) +
+                    "\" which indicates the radius used in previous filter (bm3d.VBasic or bm3d.VFinal).");
+    int proc[3] = {1, 1, 1};
+    if (s.props.v_process[0] < 0 || s.props.v_process[1] < 0 || s.props.v_process[2] < 0) {
+        log_warning("bm3d.VAggregate - warning: No frame property \"BM3D_V_process\" exists in the input frame. "
+                    "Make sure you call bm3d.VAggregate next to bm3d.VBasic or bm3d.VFinal.");
+    } else {
+        for (int p = 0; p < 3; ++p) proc[p] = (int)s.props.v_process[p];
+    }
+    const bool gray_path = vi.color_family == BM3D_CF_GRAY || (!proc[1] && !proc[2]);
+    if (P == 1 || gray_path) proc[1] = proc[2] = 0;
+    const bool full = s.props.bm3d_opp == 1 || s.props.range < 0 ? true : s.props.range != 0;
+    const hipStream_t st = f->stream;
+    int rc;
+    VAggArgs a{};
+    a.W = W;
+    a.H = H;
+    a.nframes = T;
+    a.cur = cur;
+    a.radius = r;
+    const int out_bits = f->sample ? 0 : 16;
+    const int ob = f->sample ? 4 : 2;
+    a.out_bits = out_bits;
+    const bool dev_out = d.mem == 1;
+    for (int p = 0; p < 3; ++p) {
+        a.process[p] = proc[p];
+        if (!proc[p]) continue;
+        for (int t = 0; t < T; ++t) {
+            const void* ptr;
+            size_t pe;
+            if ((rc = device_plane(src[t], p, W, vi.height, 4, f->in_raw[0][3 * t + p], st, ptr, pe))) return rc;
+            a.stacks[p * kMaxStackFrames + t] = static_cast<const float*>(ptr);
+            a.stack_pitch[p] = pe;
+        }
+        if (out_bits) float2int_coefs(16, full, p > 0, a.gain[p], a.offset[p], a.lo[p], a.hi[p]);
+        if (dev_out) {
+            a.dst[p] = d.data[p];
+            a.dst_pitch[p] = (size_t)d.stride[p] / ob;
+        } else {
+            if ((rc = f->out_stage[p].ensure((size_t)W * H * ob))) return rc;
+            a.dst[p] = f->out_stage[p].p;
+            a.dst_pitch[p] = W;
+        }
+    }
+    prof_begin(f, kTVAggregate, "k_vagg");
+    HIP_TRY(launch_vagg(a, st));
+    prof_end(f, kTVAggregate);
+    if (!dev_out)
+        for (int p = 0; p < 3; ++p)
+            if (proc[p])
+                HIP_TRY(hipMemcpy2DAsync(d.data[p], d.stride[p], a.dst[p], (size_t)W * ob, (size_t)W * ob, H,
+                                         hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    prof_collect(f);
+    d.props = s.props;
+    d.props.v_radius = -1;
+    d.props.v_process[0] = d.props.v_process[1] = d.props.v_process[2] = -1;
+    return BM3D_OK;
+}
+
 }  // namespace
 
 int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst) {
@@ -487,6 +778,11 @@ int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     case kBasic:
     case kFinal:
         return run_bm3d(f, src[0], ref, *dst);
+    case kVBasic:
+    case kVFinal:
+        return run_vbm3d(f, n, src, ref, *dst);
+    case kVAggregate:
+        return run_vaggregate(f, n, src, *dst);
     default:
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: function not implemented yet");
     }

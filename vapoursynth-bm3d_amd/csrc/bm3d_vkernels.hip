@@ -1,0 +1,396 @@
+// V-BM3D kernels: spatio-temporal block matching with predictive search,
+// aggregation into the per-frame numerator/denominator stack, and the
+// VAggregate division.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bm3d_device.h"
+#include "bm3d_kernels.h"
+#include "bm3d_params.h"
+
+namespace bm3d {
+
+// ---------------------------------------------------------------------------
+// K11: V-BM3D block matching, one wave per reference block
+// ---------------------------------------------------------------------------
+//
+// VBM3D_Process_Base::BlockMatching (source/VBM3D_Base.cpp:540-659):
+// full search in the current frame (self first), then for the frames
+// cur-1 ... 0 and cur+1 ... T-1 a predictive search over the (y, x)-sorted
+// union of +-ps_range windows around the best ps_num matches of the
+// previously searched frame in that direction (Block::GenSearchPos,
+// include/Block.h:771-827); each frame keeps its G best, the lists are
+// concatenated and, when longer than G, cut to the self entry plus the G-1
+// smallest keys.  Ties are ordered by search / concatenation index.
+
+constexpr int kVMatchWaves = 2;
+
+struct VMatchLayout {
+    int max_cand, ps_cap, cat_cap, G;
+    __host__ __device__ int words() const {
+        // keys[max(max_cand, ps_cap)] + sel (4G) + ps raw/flag/sorted
+        // (3 ps_cap) + concatenated key/pos/frame (3 cat_cap) + seeds (G)
+        int m = max_cand > ps_cap ? max_cand : ps_cap;
+        return m + 4 * G + 3 * ps_cap + 3 * cat_cap + G;
+    }
+};
+
+// Sorted, de-duplicated union of the search windows around seeds[0..ns).
+// raw/out hold packed (y << 16) | x; returns the count (wave-uniform).
+__device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, int B, int* raw, int* flag,
+                              int* out) {
+    const int lane = lane_id();
+    const int range = p.ps_range / p.ps_step * p.ps_step;
+    const int side = 2 * (range / p.ps_step) + 1;
+    // window of each seed (Block::AddSearchPos): row-major lattice
+    int nraw = 0;
+    for (int s = 0; s < ns; ++s) {
+        int sp = seeds[s];
+        int sy = sp >> 16, sx = sp & 0xffff;
+        int l = search_lo(sx, range, p.ps_step), r = search_hi(sx, p.width - B, range, p.ps_step);
+        int t = search_lo(sy, range, p.ps_step), b = search_hi(sy, p.height - B, range, p.ps_step);
+        int nc = (r - l) / p.ps_step + 1, nr = (b - t) / p.ps_step + 1;
+        for (int e = lane; e < nc * nr; e += 64) {
+            int ry = e / nc;
+            raw[nraw + e] = ((t + ry * p.ps_step) << 16) | (l + (e - ry * nc) * p.ps_step);
+        }
+        nraw += nc * nr;
+    }
+    (void)side;
+    wave_sync_lds();
+    // mark first occurrences, then place each at its rank among distinct values
+    for (int e0 = 0; e0 < nraw; e0 += 64) {
+        int e = e0 + lane;
+        if (e < nraw) {
+            int v = raw[e];
+            int first = 1;
+            for (int q = 0; q < e; ++q)
+                if (raw[q] == v) {
+                    first = 0;
+                    break;
+                }
+            flag[e] = first;
+        }
+    }
+    wave_sync_lds();
+    int total = 0;
+    for (int e0 = 0; e0 < nraw; e0 += 64) {
+        int e = e0 + lane;
+        bool first = e < nraw && flag[e];
+        if (first) {
+            int v = raw[e], rank = 0;
+            for (int q = 0; q < nraw; ++q) rank += (flag[q] && raw[q] < v);
+            out[rank] = v;
+        }
+        total += __popcll(__ballot(first));
+    }
+    wave_sync_lds();
+    return total;
+}
+
+template <int BT>
+__global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
+    extern __shared__ uint32_t vsm[];
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int b = blockIdx.x * kVMatchWaves + wave;
+    const StageParams& p = a.p;
+    if (b >= a.nb) return;
+    const int B = BT > 0 ? BT : p.block;
+    const int G = p.group;
+    VMatchLayout L{a.max_cand, a.ps_cap, a.cat_cap, G};
+    uint32_t* base = vsm + (size_t)wave * (L.words() + (BT > 0 ? 0 : B * B));
+    const int kcap = a.max_cand > a.ps_cap ? a.max_cand : a.ps_cap;
+    uint32_t* keys = base;
+    uint32_t* sel_key = keys + kcap;
+    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);
+    int* ps_raw = sel_idx + 2 * G;
+    int* ps_flag = ps_raw + a.ps_cap;
+    int* ps_pos = ps_flag + a.ps_cap;
+    uint32_t* cat_key = reinterpret_cast<uint32_t*>(ps_pos + a.ps_cap);
+    int* cat_pos = reinterpret_cast<int*>(cat_key + a.cat_cap);
+    int* cat_frm = cat_pos + a.cat_cap;
+    int* seeds = cat_frm + a.cat_cap;
+    float* rbl = reinterpret_cast<float*>(seeds + G);
+
+    const int bj = b / p.nx, bi = b - bj * p.nx;
+    const int j = grid_pos(bj, p.block_step, p.height - B);
+    const int i = grid_pos(bi, p.block_step, p.width - B);
+    int* cnt_out = a.count + b;
+    int* pos_out = a.pos + (size_t)b * G;
+    int* frm_out = a.frame + (size_t)b * G;
+    float* key_out = a.key + (size_t)b * G;
+    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
+        if (lane == 0) {
+            *cnt_out = 1;
+            pos_out[0] = (j << 16) | i;
+            frm_out[0] = a.cur;
+            key_out[0] = 0.0f;
+        }
+        return;
+    }
+    const float* cur_plane = a.planes[a.cur];
+    float rb_reg[BT > 0 ? BT * BT : 1];
+    const float* rb = rbl;
+    if constexpr (BT > 0) {
+#pragma unroll
+        for (int e = 0; e < BT * BT; ++e) rb_reg[e] = cur_plane[(size_t)(j + e / BT) * p.pitch + i + e % BT];
+        rb = rb_reg;
+    } else {
+        for (int e = lane; e < B * B; e += 64) rbl[e] = cur_plane[(size_t)(j + e / B) * p.pitch + i + e % B];
+        wave_sync_lds();
+    }
+    auto ssd_at = [&](const float* plane, int y, int x) {
+        const float* q = plane + (size_t)y * p.pitch + x;
+        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, p.pitch);
+        else return ssd_generic(rb, q, p.pitch, B);
+    };
+
+    // --- current frame: full search with the self entry first
+    const int l = search_lo(i, p.range, p.step);
+    const int r = search_hi(i, p.width - B, p.range, p.step);
+    const int t = search_lo(j, p.range, p.step);
+    const int bt = search_hi(j, p.height - B, p.range, p.step);
+    const int ncols = (r - l) / p.step + 1;
+    const int nc = ncols * ((bt - t) / p.step + 1);
+    int nacc = 0;
+    for (int c0 = 0; c0 < nc; c0 += 64) {
+        int c = c0 + lane;
+        uint32_t kv = 0xFFFFFFFFu;
+        if (c < nc) {
+            int cy = c / ncols;
+            int y = t + cy * p.step, x = l + (c - cy * ncols) * p.step;
+            if (!(y == j && x == i)) {
+                float dist = ssd_at(cur_plane, y, x);
+                if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+            }
+            keys[c] = kv;
+        }
+        nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
+    }
+    wave_sync_lds();
+    int k = nacc < G - 1 ? nacc : G - 1;
+    wave_select(keys, nc, nacc, k, sel_key, sel_idx);
+    int ncat = 1 + k;
+    if (lane == 0) {
+        cat_key[0] = 0u;
+        cat_pos[0] = (j << 16) | i;
+        cat_frm[0] = a.cur;
+    }
+    for (int e = lane; e < k; e += 64) {
+        int c = sel_idx[k + e];
+        int cy = c / ncols;
+        cat_key[1 + e] = sel_key[k + e];
+        cat_pos[1 + e] = ((t + cy * p.step) << 16) | (l + (c - cy * ncols) * p.step);
+        cat_frm[1 + e] = a.cur;
+    }
+    wave_sync_lds();
+    const int ncur_seeds = ncat < p.ps_num ? ncat : p.ps_num;
+
+    // --- predictive search, backward then forward
+    for (int dir = -1; dir <= 1; dir += 2) {
+        int ns = ncur_seeds;
+        for (int e = lane; e < ns; e += 64) seeds[e] = cat_pos[e];
+        wave_sync_lds();
+        for (int f = a.cur + dir; f >= 0 && f < a.nframes; f += dir) {
+            int np_ = gen_search_pos(seeds, ns, p, B, ps_raw, ps_flag, ps_pos);
+            const float* plane = a.planes[f];
+            int acc = 0;
+            for (int c0 = 0; c0 < np_; c0 += 64) {
+                int c = c0 + lane;
+                uint32_t kv = 0xFFFFFFFFu;
+                if (c < np_) {
+                    int pk = ps_pos[c];
+                    float dist = ssd_at(plane, pk >> 16, pk & 0xffff);
+                    if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+                    keys[c] = kv;
+                }
+                acc += __popcll(__ballot(kv != 0xFFFFFFFFu));
+            }
+            wave_sync_lds();
+            int kf = acc < G ? acc : G;
+            wave_select(keys, np_, acc, kf, sel_key, sel_idx);
+            for (int e = lane; e < kf; e += 64) {
+                cat_key[ncat + e] = sel_key[kf + e];
+                cat_pos[ncat + e] = ps_pos[sel_idx[kf + e]];
+                cat_frm[ncat + e] = f;
+            }
+            wave_sync_lds();
+            // seeds of the next frame: the best ps_num matches of this one
+            ns = kf < p.ps_num ? kf : p.ps_num;
+            for (int e = lane; e < ns; e += 64) seeds[e] = cat_pos[ncat + e];
+            ncat += kf;
+            wave_sync_lds();
+        }
+    }
+
+    // --- final cut (VBM3D_Base.cpp:652-656): keep order when within G
+    if (ncat <= G) {
+        for (int e = lane; e < ncat; e += 64) {
+            pos_out[e] = cat_pos[e];
+            frm_out[e] = cat_frm[e];
+            key_out[e] = __uint_as_float(cat_key[e]);
+        }
+        if (lane == 0) *cnt_out = ncat;
+        return;
+    }
+    // select G-1 of entries 1..ncat-1 by (key, concatenation index)
+    int m = ncat - 1;
+    wave_select(cat_key + 1, m, m, G - 1, sel_key, sel_idx);
+    if (lane == 0) {
+        *cnt_out = G;
+        pos_out[0] = cat_pos[0];
+        frm_out[0] = cat_frm[0];
+        key_out[0] = __uint_as_float(cat_key[0]);
+    }
+    for (int e = lane; e < G - 1; e += 64) {
+        int c = 1 + sel_idx[G - 1 + e];
+        pos_out[1 + e] = cat_pos[c];
+        frm_out[1 + e] = cat_frm[c];
+        key_out[1 + e] = __uint_as_float(sel_key[G - 1 + e]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// V-BM3D aggregation into the stacked num/den slots
+// ---------------------------------------------------------------------------
+//
+// VBM3D_Process_Base::Kernel writes each member into the numerator and
+// denominator planes of its own frame (BlockGroup Pos3 AddTo / CountTo,
+// include/Block.h:1283-1303, 1368-1387) in reference-block scan order, then
+// member order.  One wave per (8x8 tile, frame): it walks the reference blocks
+// that can reach the tile in that frame and accumulates the members of that
+// frame in exactly that order.  Output slot 2*(radius + f - cur) (num) and
+// +1 (den) of the output stack (VBM3D_Base.h:284-292).
+
+__global__ __launch_bounds__(256) void k_vstack(VStackArgs a) {
+    const int lane = lane_id();
+    const int tiles_x = (a.p.width + 7) >> 3, tiles_y = (a.p.height + 7) >> 3;
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wid >= tiles_x * tiles_y * a.nframes) return;
+    const int fr = wid / (tiles_x * tiles_y);
+    const int tile = wid - fr * tiles_x * tiles_y;
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    const int B = a.p.block, G = a.p.group, BB = B * B;
+    const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+    const bool valid = px < a.p.width && py < a.p.height;
+    const int y0 = ty * 8, x0 = tx * 8;
+    const int dist = fr > a.cur ? fr - a.cur : a.cur - fr;
+    const int reach = a.p.range + dist * (a.p.ps_range / a.p.ps_step * a.p.ps_step);
+    const int lo_y = y0 - B + 1 - reach, hi_y = y0 + 7 + reach;
+    const int lo_x = x0 - B + 1 - reach, hi_x = x0 + 7 + reach;
+    const int s = a.p.block_step;
+    int a_lo = lo_y > 0 ? lo_y / s - 1 : 0, a_hi = hi_y / s + 1;
+    int c_lo = lo_x > 0 ? lo_x / s - 1 : 0, c_hi = hi_x / s + 1;
+    a_lo = a_lo < 0 ? 0 : a_lo;
+    c_lo = c_lo < 0 ? 0 : c_lo;
+    a_hi = a_hi > a.p.ny - 1 ? a.p.ny - 1 : a_hi;
+    c_hi = c_hi > a.p.nx - 1 ? a.p.nx - 1 : c_hi;
+    bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
+    float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
+    for (int ar = a_lo; ar <= a_hi; ++ar) {
+        for (int ac = c_lo; ac <= c_hi; ++ac) {
+            const int b = ar * a.p.nx + ac;
+            int n = a.count[b];
+            n = uniform(n < G ? n : G);
+            for (int z0 = 0; z0 < n; z0 += 64) {
+                int z = z0 + lane;
+                int pk = 0;
+                bool ov = false;
+                if (z < n && a.frame[(size_t)b * G + z] == fr) {
+                    pk = a.pos[(size_t)b * G + z];
+                    int qy = pk >> 16, qx = pk & 0xffff;
+                    ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
+                }
+                uint64_t m = __ballot(ov);
+                while (m) {
+                    int zl = __builtin_ctzll(m);
+                    m &= m - 1;
+                    int q = __builtin_amdgcn_readlane(pk, zl);
+                    int dy = py - (q >> 16), dx = px - (q & 0xffff);
+                    bool cov = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
+                    size_t off = ((size_t)b * G + z0 + zl) * BB + (size_t)(dy * B + dx);
+#pragma unroll
+                    for (int pl = 0; pl < 3; ++pl) {
+                        if (!proc[pl]) continue;
+                        if (cov) {
+                            num[pl] = num[pl] + a.contrib[pl][off];
+                            den[pl] = den[pl] + a.denw[pl][b];
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const int slot = 2 * (a.p.radius + fr - a.cur);
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!proc[pl]) continue;
+        float* basep = a.dst[pl];
+        basep[((size_t)slot * a.p.height + py) * a.dst_pitch[pl] + px] = num[pl];
+        basep[((size_t)(slot + 1) * a.p.height + py) * a.dst_pitch[pl] + px] = den[pl];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K12: VAggregate (source/VAggregate.cpp:93-160)
+// ---------------------------------------------------------------------------
+//
+// out = (sum_m num_m) / (sum_m den_m) over the neighbours m = n-b .. n+f in
+// ascending order, slot radius - (m - n) of neighbour m's stack, then the
+// Float2Int conversion.  Streaming, one thread per pixel.
+
+__global__ void k_vagg(VAggArgs a) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int pl = blockIdx.z;
+    if (x >= a.W || y >= a.H || !a.process[pl]) return;
+    float num = 0.0f, den = 0.0f;
+    for (int i = 0; i < a.nframes; ++i) {
+        const int o = a.radius + a.cur - i;
+        const float* st = a.stacks[pl * kMaxStackFrames + i];
+        const size_t sp = a.stack_pitch[pl];
+        num = num + st[((size_t)(2 * o) * a.H + y) * sp + x];
+        den = den + st[((size_t)(2 * o + 1) * a.H + y) * sp + x];
+    }
+    float v = num / den;
+    if (a.out_bits != 0) {
+        v = v * a.gain[pl] + a.offset[pl];
+        v = clipf(v, a.lo[pl], a.hi[pl]);
+    }
+    store_sample(a.dst[pl], a.out_bits, (size_t)y * a.dst_pitch[pl] + x, v);
+}
+
+// ---------------------------------------------------------------------------
+
+size_t vmatch_lds_bytes(int max_cand, int ps_cap, int cat_cap, int B, int G) {
+    VMatchLayout L{max_cand, ps_cap, cat_cap, G};
+    return sizeof(uint32_t) * (size_t)kVMatchWaves * (L.words() + (B == 8 ? 0 : B * B));
+}
+
+hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st) {
+    size_t lds = vmatch_lds_bytes(a.max_cand, a.ps_cap, a.cat_cap, a.p.block, a.p.group);
+    int blocks = (a.nb + kVMatchWaves - 1) / kVMatchWaves;
+    if (blocks == 0) return hipSuccess;
+    if (a.p.block == 8)
+        hipLaunchKernelGGL(k_vmatch<8>, dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
+    else
+        hipLaunchKernelGGL(k_vmatch<0>, dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vstack(const VStackArgs& a, hipStream_t st) {
+    int tiles = ((a.p.width + 7) / 8) * ((a.p.height + 7) / 8) * a.nframes;
+    hipLaunchKernelGGL(k_vstack, dim3((tiles + 3) / 4), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vagg(const VAggArgs& a, hipStream_t st) {
+    dim3 grid((a.W + 255) / 256, a.H, 3);
+    hipLaunchKernelGGL(k_vagg, grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace bm3d
