@@ -124,6 +124,41 @@ __device__ inline float ssd_generic(const float* rb, const float* __restrict__ q
     return dist;
 }
 
+// Sorts the k (<= 256) entries sel_key/sel_idx[0..k) by (key, index) into
+// sel_key/sel_idx[k..2k) by counting, for each entry, the entries before it.
+__device__ inline void rank_sort_selected(uint32_t* sel_key, int* sel_idx, int k) {
+    const int lane = lane_id();
+    uint32_t rk[4];
+    int ri[4], rr[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int e = lane + 64 * s;
+        rk[s] = e < k ? sel_key[e] : 0xFFFFFFFFu;
+        ri[s] = e < k ? sel_idx[e] : 0x7FFFFFFF;
+        rr[s] = 0;
+    }
+    for (int e2 = 0; e2 < k; ++e2) {
+        uint32_t k2 = sel_key[e2];
+        int i2 = sel_idx[e2];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) rr[s] += (k2 < rk[s]) || (k2 == rk[s] && i2 < ri[s]);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        int e = lane + 64 * s;
+        if (e < k) {
+            sel_key[k + rr[s]] = rk[s];
+            sel_idx[k + rr[s]] = ri[s];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // exact top-k selection on a wave
 // ---------------------------------------------------------------------------
@@ -188,36 +223,7 @@ __device__ inline void wave_select(const uint32_t* keys, int nc, int naccepted, 
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // rank sort of the (<= 255) selected entries by (key, index)
-    uint32_t rk[4];
-    int ri[4], rr[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int e = lane + 64 * s;
-        rk[s] = e < k ? sel_key[e] : 0xFFFFFFFFu;
-        ri[s] = e < k ? sel_idx[e] : 0x7FFFFFFF;
-        rr[s] = 0;
-    }
-    for (int e2 = 0; e2 < k; ++e2) {
-        uint32_t k2 = sel_key[e2];
-        int i2 = sel_idx[e2];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) rr[s] += (k2 < rk[s]) || (k2 == rk[s] && i2 < ri[s]);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        int e = lane + 64 * s;
-        if (e < k) {
-            sel_key[k + rr[s]] = rk[s];
-            sel_idx[k + rr[s]] = ri[s];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    rank_sort_selected(sel_key, sel_idx, k);
 }
 
 __device__ __forceinline__ void wave_sync_lds() {

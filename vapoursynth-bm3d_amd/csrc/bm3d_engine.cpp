@@ -447,6 +447,13 @@ int bm3d_init(int device) {
     if (g_dev.rcp_dev) (void)hipFree(g_dev.rcp_dev);
     HIP_TRY(hipMalloc(&g_dev.rcp_dev, tbl.size() * sizeof(float)));
     HIP_TRY(hipMemcpy(g_dev.rcp_dev, tbl.data(), tbl.size() * sizeof(float), hipMemcpyHostToDevice));
+    // 8-point DCT coefficients of the B = 8 kernels (same expression as the
+    // length-N tables of upload_coefs)
+    float c8[64];
+    for (int k = 0; k < 8; ++k)
+        for (int j = 0; j < 8; ++j)
+            c8[k * 8 + j] = (float)(2.0 * std::cos(M_PI * (double)k * (double)(2 * j + 1) / (2.0 * 8.0)));
+    HIP_TRY(upload_dct8(c8));
     g_dev.ready = true;
     return BM3D_OK;
 }

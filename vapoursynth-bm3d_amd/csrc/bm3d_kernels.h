@@ -109,6 +109,7 @@ hipError_t launch_vagg(const VAggArgs& a, hipStream_t st);
 size_t match_wave_words(int max_cand, int B, int G);
 size_t match_lds_bytes(int max_cand, int B, int G);
 size_t filter_lds_bytes(int B, int G);
+hipError_t upload_dct8(const float* c64);
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
                         float* key, hipStream_t st);
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st);

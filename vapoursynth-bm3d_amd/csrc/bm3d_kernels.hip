@@ -131,6 +131,171 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
     }
 }
 
+// Fast path for B = 8 and bm_step = 1: the search window is staged in LDS
+// and each lane scores a horizontal strip of 8 neighbouring candidates, so a
+// window row of 15 samples feeds 64 differences (15 LDS reads per 8x8 SSD
+// instead of 64).  The per-candidate arithmetic is exactly ssd_fixed<8>'s.
+// Keys stay in registers; selection is a radix descent on ballot counts.
+template <int KMAX>
+__global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __restrict__ plane, StageParams p, int nb,
+                                                             int wave_words, int* __restrict__ out_count,
+                                                             int* __restrict__ out_pos, float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    const int b = blockIdx.x * kMatchWaves + wave;
+    if (b >= nb) return;
+    constexpr int B = 8;
+    const int G = p.group;
+    float* win = reinterpret_cast<float*>(smem + (size_t)wave * wave_words);
+    const int bj = b / p.nx, bi = b - bj * p.nx;
+    const int j = grid_pos(bj, p.block_step, p.height - B);
+    const int i = grid_pos(bi, p.block_step, p.width - B);
+    int* cnt_out = out_count + b;
+    int* pos_out = out_pos + (size_t)b * G;
+    float* key_out = out_key + (size_t)b * G;
+    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
+        if (lane == 0) {
+            *cnt_out = 1;
+            pos_out[0] = (j << 16) | i;
+            key_out[0] = 0.0f;
+        }
+        return;
+    }
+    const int l = search_lo(i, p.range, 1), r = search_hi(i, p.width - B, p.range, 1);
+    const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
+    const int ncols = r - l + 1, nrows = bt - t + 1;
+    const int WW = ncols + 7 + 8;  // window row incl. slack for the last strip
+    const int WH = nrows + 7;
+    for (int e = lane; e < WH * WW; e += 64) {
+        int wy = e / WW, wx = e - wy * WW;
+        int gx = l + wx;
+        win[e] = gx < p.width ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
+    }
+    uint32_t* sel_key = smem + (size_t)wave * wave_words + WH * WW;
+    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);
+    wave_sync_lds();
+    float rb[64];
+    const int oy = j - t, ox = i - l;
+#pragma unroll
+    for (int e = 0; e < 64; ++e) rb[e] = win[(oy + e / 8) * WW + ox + e % 8];
+
+    const int nstr = (ncols + 7) >> 3;
+    const int tasks = nrows * nstr;
+    uint32_t keys[KMAX * 8];
+    int naccepted = 0;
+#pragma unroll
+    for (int tk = 0; tk < KMAX; ++tk) {
+        const int task = lane + 64 * tk;
+        const bool active = task < tasks;
+        const int cy = active ? task / nstr : 0;
+        const int sx = active ? (task - cy * nstr) * 8 : 0;
+        float s[8][4];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s[k][c] = 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            const float* wr = win + (cy + rr) * WW + sx;
+            float row[15];
+#pragma unroll
+            for (int q = 0; q < 15; ++q) row[q] = wr[q];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float d = rb[rr * 8 + c] - row[k + c];
+                    s[k][c] = s[k][c] + d * d;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float d = rb[rr * 8 + 4 + c] - row[k + 4 + c];
+                    s[k][c] = s[k][c] + d * d;
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float dist = 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
+            const int cx = sx + k;
+            const bool valid = active && cx < ncols && !(cy == oy && cx == ox);
+            uint32_t kv = 0xFFFFFFFFu;
+            if (valid && dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+            keys[tk * 8 + k] = kv;
+            naccepted += __popcll(__ballot(kv != 0xFFFFFFFFu));
+        }
+    }
+    // candidate index of register slot (tk, k): cy * ncols + cx (row-major lattice order)
+    auto cand_idx = [&](int tk, int k) {
+        int task = lane + 64 * tk;
+        int cy = task / nstr;
+        return cy * ncols + (task - cy * nstr) * 8 + k;
+    };
+    const int kk = naccepted < G - 1 ? naccepted : G - 1;
+    const bool cut = naccepted > kk;
+    uint32_t thr = 0xFFFFFFFEu;
+    int ithr = 0x7FFFFFFF;  // among keys == thr, admit candidate indices < ithr
+    if (cut) {
+        uint32_t tv = 0;
+        for (int bit = 31; bit >= 0; --bit) {
+            const uint32_t trial = tv | (1u << bit);
+            int cnt = 0;
+#pragma unroll
+            for (int q = 0; q < KMAX * 8; ++q) cnt += __popcll(__ballot(keys[q] < trial));
+            if (cnt < kk) tv = trial;
+        }
+        int less = 0, eq = 0;
+#pragma unroll
+        for (int q = 0; q < KMAX * 8; ++q) {
+            less += __popcll(__ballot(keys[q] < tv));
+            eq += __popcll(__ballot(keys[q] == tv));
+        }
+        thr = tv;
+        const int need = kk - less;
+        if (eq > need) {
+            // equal keys at the cut: the earliest candidates win
+            int it = 0;
+            for (int bit = 20; bit >= 0; --bit) {
+                const int trial = it | (1 << bit);
+                int cnt = 0;
+#pragma unroll
+                for (int q = 0; q < KMAX * 8; ++q)
+                    cnt += __popcll(__ballot(keys[q] == tv && cand_idx(q >> 3, q & 7) < trial));
+                if (cnt < need) it = trial;
+            }
+            ithr = it + 1;  // indices <= it
+        }
+    }
+    int base = 0;
+#pragma unroll
+    for (int q = 0; q < KMAX * 8; ++q) {
+        const uint32_t kv = keys[q];
+        const int ci = cand_idx(q >> 3, q & 7);
+        const bool sel = cut ? (kv < thr || (kv == thr && ci < ithr)) : (kv != 0xFFFFFFFFu);
+        const uint64_t m = __ballot(sel);
+        const int at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (sel) {
+            sel_key[at] = kv;
+            sel_idx[at] = ci;
+        }
+        base += __popcll(m);
+    }
+    wave_sync_lds();
+    rank_sort_selected(sel_key, sel_idx, kk);
+    if (lane == 0) {
+        *cnt_out = kk + 1;
+        pos_out[0] = (j << 16) | i;
+        key_out[0] = 0.0f;
+    }
+    for (int e = lane; e < kk; e += 64) {
+        const int c = sel_idx[kk + e];
+        const int cy = c / ncols;
+        pos_out[1 + e] = ((t + cy) << 16) | (l + c - cy * ncols);
+        key_out[1 + e] = __uint_as_float(sel_key[kk + e]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K3-K7: gather, 3D DCT, shrinkage, inverse 3D DCT, weights
 // ---------------------------------------------------------------------------
@@ -342,6 +507,249 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
     }
 }
 
+// Fast path for B = 8, G <= GMAX: the group sits in LDS for the x and y
+// passes (one lane per line, 8-point even/odd chains with coefficients from
+// constant memory) while the z pass, the shrinkage and the inverse z pass run
+// in registers with one lane per (y, x) column.  Every chain is the one
+// dct2_line / dct3_line evaluates, so results equal the generic path's.
+__constant__ float c_dct8[64];  // float(2 cos(pi k (2j+1) / 16)) at [k * 8 + j]
+
+__device__ __forceinline__ void dct8_fwd(const float* in, float* out) {
+    float u[4], v[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        u[j] = in[j] + in[7 - j];
+        v[j] = in[j] - in[7 - j];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const float* w = (k & 1) ? v : u;
+        float acc = c_dct8[k * 8] * w[0];
+#pragma unroll
+        for (int j = 1; j < 4; ++j) acc = __builtin_fmaf(c_dct8[k * 8 + j], w[j], acc);
+        out[k] = acc;
+    }
+}
+
+__device__ __forceinline__ void dct8_inv(const float* in, float* out) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float e = in[0];
+#pragma unroll
+        for (int k = 2; k < 8; k += 2) e = __builtin_fmaf(c_dct8[k * 8 + j], in[k], e);
+        float o = c_dct8[8 + j] * in[1];
+#pragma unroll
+        for (int k = 3; k < 8; k += 2) o = __builtin_fmaf(c_dct8[k * 8 + j], in[k], o);
+        out[j] = e + o;
+        out[7 - j] = e - o;
+    }
+}
+
+// x (rows) then y (columns) passes of the n slices in LDS (row stride 9).
+template <bool FWD>
+__device__ __forceinline__ void passes_xy(float* g, int n) {
+    const int lane = lane_id();
+    float in[8], out[8];
+    // x: lines (z, y)
+    if (FWD) {
+        for (int L = lane; L < n * 8; L += 64) {
+            float* row = g + L * 9;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) in[q] = row[q];
+            dct8_fwd(in, out);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) row[q] = out[q];
+        }
+        wave_sync_lds();
+    }
+    // y: lines (z, x)
+    for (int L = lane; L < n * 8; L += 64) {
+        float* col = g + (L >> 3) * 72 + (L & 7);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) in[q] = col[q * 9];
+        if (FWD) dct8_fwd(in, out);
+        else dct8_inv(in, out);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) col[q * 9] = out[q];
+    }
+    wave_sync_lds();
+    if (!FWD) {
+        for (int L = lane; L < n * 8; L += 64) {
+            float* row = g + L * 9;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) in[q] = row[q];
+            dct8_inv(in, out);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) row[q] = out[q];
+        }
+        wave_sync_lds();
+    }
+}
+
+// Forward z transform of the lane's column (y, x): g[z*72 + off] -> X[k].
+template <int GMAX>
+__device__ __forceinline__ void zfwd(const float* g, int off, int n, const float* __restrict__ cz, float* X) {
+    const int h = n >> 1;
+    float u[GMAX / 2 > 0 ? GMAX / 2 : 1], v[GMAX / 2 > 0 ? GMAX / 2 : 1];
+#pragma unroll
+    for (int j = 0; j < GMAX / 2; ++j) {
+        if (j < h) {
+            float a0 = g[j * 72 + off], b0 = g[(n - 1 - j) * 72 + off];
+            u[j] = a0 + b0;
+            v[j] = a0 - b0;
+        }
+    }
+    const float mid = (n & 1) ? g[h * 72 + off] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < GMAX; ++k) {
+        if (k < n) {
+            float acc;
+            if ((k & 1) == 0) {
+                if (h > 0) {
+                    acc = cz[k * n] * u[0];
+#pragma unroll
+                    for (int j = 1; j < GMAX / 2; ++j)
+                        if (j < h) acc = __builtin_fmaf(cz[k * n + j], u[j], acc);
+                    if (n & 1) acc = __builtin_fmaf(cz[k * n + h], mid, acc);
+                } else {
+                    acc = cz[0] * mid;
+                }
+            } else {
+                acc = cz[k * n] * v[0];
+#pragma unroll
+                for (int j = 1; j < GMAX / 2; ++j)
+                    if (j < h) acc = __builtin_fmaf(cz[k * n + j], v[j], acc);
+            }
+            X[k] = acc;
+        }
+    }
+}
+
+// Inverse z transform of X[k] into the lane's column of g.
+template <int GMAX>
+__device__ __forceinline__ void zinv(float* g, int off, int n, const float* __restrict__ cz, const float* X) {
+    const int h = n >> 1;
+#pragma unroll
+    for (int j = 0; j < (GMAX + 1) / 2; ++j) {
+        if (j < h + (n & 1)) {
+            float e = X[0];
+#pragma unroll
+            for (int k = 2; k < GMAX; k += 2)
+                if (k < n) e = __builtin_fmaf(cz[k * n + j], X[k], e);
+            float o = 0.0f;
+            if (j < h) {
+                o = cz[n + j] * X[1];
+#pragma unroll
+                for (int k = 3; k < GMAX; k += 2)
+                    if (k < n) o = __builtin_fmaf(cz[k * n + j], X[k], o);
+                g[j * 72 + off] = e + o;
+                g[(n - 1 - j) * 72 + off] = e - o;
+            } else {
+                g[j * 72 + off] = e;
+            }
+        }
+    }
+}
+
+template <int GMAX, bool WIENER>
+__global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
+    extern __shared__ float fsm[];
+    const int g_id = a.b0 + blockIdx.x;
+    if (g_id >= a.nb) return;
+    const int lane = lane_id();
+    const int G = a.p.group;
+    const int n = uniform(a.count[g_id] < G ? a.count[g_id] : G);
+    float* gs = fsm;                       // [GMAX][8][9]
+    float* gr = gs + GMAX * 72;            // [GMAX][8][9] (Wiener), then w^2 chains
+    int* zyx = reinterpret_cast<int*>(gr + (WIENER ? GMAX * 72 : 0));
+    for (int z = lane; z < n; z += 64) {
+        int pk = a.pos[(size_t)g_id * G + z];
+        zyx[3 * z] = a.frame ? a.frame[(size_t)g_id * G + z] : 0;
+        zyx[3 * z + 1] = pk >> 16;
+        zyx[3 * z + 2] = pk & 0xffff;
+    }
+    wave_sync_lds();
+    const int y = lane >> 3, x = lane & 7;
+    const int off = y * 9 + x;
+    const float* cz = a.coef + a.coef_off[n];
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!a.process[pl]) continue;
+        const PlaneTables& tb = a.tables[pl];
+        const float* const* sp = a.src[pl];
+        const float* const* rp = a.ref[pl];
+        for (int z = 0; z < n; ++z) {
+            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
+            gs[z * 72 + off] = sp[pz][(size_t)(py + y) * a.p.pitch + px + x];
+            if (WIENER) gr[z * 72 + off] = rp[pz][(size_t)(py + y) * a.p.pitch + px + x];
+        }
+        wave_sync_lds();
+        float X[GMAX];
+        float den_w;
+        if (!WIENER) {
+            passes_xy<true>(gs, n);
+            zfwd<GMAX>(gs, off, n, cz, X);
+            int kept = 0;
+#pragma unroll
+            for (int k = 0; k < GMAX; ++k) {
+                if (k < n) {
+                    const float thr = tb.thr[n - 1][(x == 0) + (y == 0) + (k == 0)];
+                    const float s = X[k];
+                    if (s > thr || s < -thr) ++kept;
+                    else X[k] = 0.0f;
+                }
+            }
+            for (int o = 32; o >= 1; o >>= 1) kept += __shfl_xor(kept, o);
+            den_w = kept < 1 ? 1.0f : 1.0f / (float)kept;
+        } else {
+            passes_xy<true>(gr, n);
+            float R[GMAX];
+            zfwd<GMAX>(gr, off, n, cz, R);
+            passes_xy<true>(gs, n);
+            zfwd<GMAX>(gs, off, n, cz, X);
+            const float sig2 = tb.wiener_sigma_sqr[n - 1];
+            // w^2 stored chain-major: SSE2 lane c = x % 4 takes flat indices
+            // e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
+            float* w2 = gr;
+            const int c = x & 3;
+            const int ibase = y * 2 + (x >> 2);
+#pragma unroll
+            for (int k = 0; k < GMAX; ++k) {
+                if (k < n) {
+                    const float r2 = R[k] * R[k];
+                    const float w = r2 * rcp_host(a.rcp_table, a.rcp_shift, r2 + sig2);
+                    X[k] = X[k] * w;
+                    w2[c * 16 * n + k * 16 + ibase] = w * w;
+                }
+            }
+            wave_sync_lds();
+            float part = 0.0f;
+            if (lane < 4) {
+                const float4* ch = reinterpret_cast<const float4*>(w2 + lane * 16 * n);
+                for (int q = 0; q < 4 * n; ++q) {
+                    float4 v4 = ch[q];
+                    part = part + v4.x;
+                    part = part + v4.y;
+                    part = part + v4.z;
+                    part = part + v4.w;
+                }
+            }
+            const float p0 = __shfl(part, 0), p1 = __shfl(part, 1), p2 = __shfl(part, 2), p3 = __shfl(part, 3);
+            float l2 = 0.0f + (((p0 + p1) + p2) + p3);
+            l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;
+            den_w = 1.0f / l2;
+            wave_sync_lds();
+        }
+        zinv<GMAX>(gs, off, n, cz, X);
+        wave_sync_lds();
+        passes_xy<false>(gs, n);
+        const float num_w = (float)((double)den_w / tb.final_amp[n - 1]);
+        float* contrib = a.contrib[pl] + (size_t)g_id * G * 64;
+        for (int z = 0; z < n; ++z) contrib[z * 64 + lane] = gs[z * 72 + off] * num_w;
+        if (lane == 0) a.denw[pl][g_id] = den_w;
+        wave_sync_lds();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K8-K10: ordered gather aggregation, divide and output conversion
 // ---------------------------------------------------------------------------
@@ -518,12 +926,45 @@ size_t match_lds_bytes(int max_cand, int B, int G) {
     return sizeof(uint32_t) * (size_t)kMatchWaves * match_wave_words(max_cand, B, G);
 }
 
+int match8_kmax(const StageParams& p, int max_cand) {
+    // tasks = rows x strips of 8 for the largest (interior) window
+    int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
+    int tasks = side * ((side + 7) / 8);
+    return (tasks + 63) / 64;
+}
+
+size_t match8_wave_words(int max_cand, int G) {
+    int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
+    return (size_t)(side + 7) * (side + 15) + 4 * (size_t)G;
+}
+
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
                         float* key, hipStream_t st) {
-    size_t lds = match_lds_bytes(max_cand, p.block, p.group);
-    int ww = (int)match_wave_words(max_cand, p.block, p.group);
     int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
     if (blocks == 0) return hipSuccess;
+    if (p.block == 8 && p.step == 1) {
+        const int km = match8_kmax(p, max_cand);
+        const size_t ww = match8_wave_words(max_cand, p.group);
+        const size_t lds = ww * 4 * kMatchWaves;
+        if (km <= 8 && lds <= 65536) {
+#define BM3D_M8(K)                                                                                      \
+    case K:                                                                                            \
+        hipLaunchKernelGGL(k_match8<K>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, nb, (int)ww, \
+                           count, pos, key);                                                           \
+        return hipGetLastError();
+            switch (km <= 1 ? 1 : km <= 2 ? 2 : km <= 3 ? 3 : km <= 4 ? 4 : km <= 6 ? 6 : 8) {
+                BM3D_M8(1)
+                BM3D_M8(2)
+                BM3D_M8(3)
+                BM3D_M8(4)
+                BM3D_M8(6)
+                BM3D_M8(8)
+            }
+#undef BM3D_M8
+        }
+    }
+    size_t lds = match_lds_bytes(max_cand, p.block, p.group);
+    int ww = (int)match_wave_words(max_cand, p.block, p.group);
     if (p.block == 8)
         hipLaunchKernelGGL(k_match<8>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, 0, nb, max_cand,
                            ww, count, pos, key);
@@ -537,8 +978,33 @@ size_t filter_lds_bytes(int B, int G) {
     return sizeof(float) * (size_t)3 * G * B * (B + 1) + sizeof(int) * 3 * (size_t)G;
 }
 
+static int filter8_gmax(int G) { return G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 0; }
+
+size_t filter8_lds_bytes(int G, bool wiener) {
+    int gm = filter8_gmax(G);
+    return sizeof(float) * (size_t)gm * 72 * (wiener ? 2 : 1) + sizeof(int) * 3 * (size_t)gm;
+}
+
+hipError_t upload_dct8(const float* c64) { return hipMemcpyToSymbol(HIP_SYMBOL(c_dct8), c64, 64 * sizeof(float)); }
+
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
     if (a.nb == 0) return hipSuccess;
+    const int gm = filter8_gmax(a.p.group);
+    if (a.p.block == 8 && gm > 0) {
+        const size_t lds = filter8_lds_bytes(a.p.group, a.wiener);
+#define BM3D_F8(GM, W)                                                                   \
+    if (gm == GM && a.wiener == W) {                                                    \
+        hipLaunchKernelGGL((k_filter8<GM, W>), dim3(a.nb), dim3(64), lds, st, a);        \
+        return hipGetLastError();                                                        \
+    }
+        BM3D_F8(8, 0)
+        BM3D_F8(16, 0)
+        BM3D_F8(32, 0)
+        BM3D_F8(8, 1)
+        BM3D_F8(16, 1)
+        BM3D_F8(32, 1)
+#undef BM3D_F8
+    }
     size_t lds = filter_lds_bytes(a.p.block, a.p.group);
     hipLaunchKernelGGL(k_filter, dim3(a.nb), dim3(64), lds, st, a);
     return hipGetLastError();
