@@ -112,3 +112,49 @@ def test_rgb2opp_opp2rgb(gpu):
         ref2 = O.opp2rgb(ref, 0 if sample else 16, 16, True)
         _assert_equal(np.stack(back.planes), ref2, f"OPP2RGB from sample={sample}")
         assert "BM3D_OPP" not in back.props and back.props["_Matrix"] == 0
+
+
+def test_config1_640x480_basic_fast_full_frame(gpu):
+    """The reference's CPU-runnable workload: one Gray float 640x480 frame,
+    Basic profile 'fast', sigma 10: every match list and every output sample."""
+    h, w = 480, 640
+    noisy, _ = synth.gray_float(h, w, 10.0, seed=1000)
+    node = gpu.Basic(_gray_clip(gpu, noisy), profile="fast")
+    out = node.get_frame(0).planes[0]
+    f = make("Basic", gpu.GRAYS, w, h, profile="fast")
+    ref = bm3d_frame(f, gpu.GRAYS, noisy[None])[0]
+    _assert_equal(out, ref, "Basic 640x480 fast")
+    flt = node.filter
+    gy = np.zeros(h + 1, np.int32)
+    gx = np.zeros(w + 1, np.int32)
+    ny = O.lib().or_ref_grid(h, 8, 8, O._ip(gy))
+    nx = O.lib().or_ref_grid(w, 8, 8, O._ip(gx))
+    assert ny * nx == 4800
+    for b in range(0, ny * nx, 7):
+        j, i = int(gy[b // nx]), int(gx[b % nx])
+        k_ref, y_ref, x_ref = O.block_match(f, noisy, j, i)
+        k_gpu, y_gpu, x_gpu = flt.matches(b)
+        assert np.array_equal(y_gpu, y_ref) and np.array_equal(x_gpu, x_ref), (b, j, i)
+        assert np.array_equal(k_gpu.view(np.uint32), k_ref.view(np.uint32)), (b, j, i)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(block_size=4, block_step=3, group_size=8, bm_range=5),      # B < 8: scalar SSD, generic kernels
+    dict(block_size=11, block_step=6, group_size=16, bm_range=7),    # SIMD part + residue columns
+    dict(group_size=1),                                              # no matching
+    dict(group_size=64, bm_range=6),                                 # G > 32: generic transform path
+    dict(bm_step=2, bm_range=8),                                     # sparse search lattice
+])
+def test_basic_final_parameter_variants(gpu, kw):
+    h, w = 40, 48
+    noisy, clean = synth.gray_float(h, w, 15.0, seed=77)
+    for fn in ("Basic", "Final"):
+        if fn == "Basic":
+            node = gpu.Basic(_gray_clip(gpu, noisy), profile="np", **kw)
+            f = make("Basic", gpu.GRAYS, w, h, profile="np", **kw)
+            ref = bm3d_frame(f, gpu.GRAYS, noisy[None])[0]
+        else:
+            node = gpu.Final(_gray_clip(gpu, noisy), _gray_clip(gpu, clean), profile="np", **kw)
+            f = make("Final", gpu.GRAYS, w, h, profile="np", **kw)
+            ref = bm3d_frame(f, gpu.GRAYS, noisy[None], clean[None])[0]
+        _assert_equal(node.get_frame(0).planes[0], ref, f"{fn} {kw}")

@@ -464,3 +464,27 @@ def test_rcp_native_equals_table_model(oracle_lib):
     for e in (-20, -3, 5, 17):
         sc = np.array([O.lib().or_rcp(O.C.byref(fn), float(np.ldexp(v, e))) for v in xs[:200]], np.float32)
         assert np.array_equal(sc, np.ldexp(native[:200], -e).astype(np.float32))
+
+
+def test_tie_orders_differ_only_at_equal_keys(oracle_lib):
+    """The GPU orders equal keys by search index (TIES_INDEX); the reference's
+    libstdc++ partial_sort may order / cut them differently.  On the
+    640x480 Basic 'fast' workload every difference between the two match
+    lists must be explained by equal keys."""
+    H, W = 480, 640
+    noisy, _ = synth.gray_float(H, W, 10.0, seed=1000)
+    p = O.resolve("Basic", width=W, height=H, color_family="gray", profile="fast")
+    f_ref = O.make_filter(p, tie_mode=O.TIES_LIBSTDCXX)
+    f_idx = O.make_filter(p, tie_mode=O.TIES_INDEX)
+    B, s = p.block_size, p.block_step
+    differ = 0
+    for j in _grid(H, B, s)[::3]:
+        for i in _grid(W, B, s):
+            k1, y1, x1 = O.block_match(f_ref, noisy, j, i)
+            k2, y2, x2 = O.block_match(f_idx, noisy, j, i)
+            assert np.array_equal(k1, k2)          # same key sequence always
+            if not (np.array_equal(y1, y2) and np.array_equal(x1, x2)):
+                differ += 1
+                for q in np.nonzero((y1 != y2) | (x1 != x2))[0]:
+                    assert np.sum(k1 == k1[q]) > 1  # position differs only inside a run of equal keys
+    assert differ < 50

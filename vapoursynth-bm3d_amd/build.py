@@ -34,13 +34,42 @@ FLAGS = [
 ]
 
 
+def write_dct_header(path: str, nmax: int = 32) -> None:
+    """Emits float(2 cos(pi k (2j+1) / (2N))) for N = 1..nmax as exact hex
+    literals, computed with the C library's cos (Python's math.cos), so the
+    kernels can use them as instruction constants.  The library checks them
+    against the host's std::cos at bm3d_init."""
+    import math
+    import struct
+
+    def f32(v: float) -> float:
+        return struct.unpack("<f", struct.pack("<f", v))[0]
+
+    lines = ["// Generated by build.py -- do not edit.", "#pragma once", "namespace bm3d {",
+             "template <int N> struct DctC;"]
+    for N in range(1, nmax + 1):
+        vals = []
+        for k in range(N):
+            for j in range(N):
+                vals.append(f32(2.0 * math.cos(math.pi * float(k) * float(2 * j + 1) / (2.0 * float(N)))))
+        lit = ", ".join(float.hex(v) + "f" for v in vals)
+        lines.append(f"template <> struct DctC<{N}> {{ static constexpr float v[{N * N}] = {{{lit}}}; }};")
+    lines.append(f"constexpr int kDctCMax = {nmax};")
+    lines.append("}  // namespace bm3d")
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    os.replace(tmp, path)
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
+    write_dct_header(os.path.join(OUT_DIR, "bm3d_dct_coefs.h"))
     objs = []
     for src in SOURCES:
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *FLAGS, "-I", OUT_DIR, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(shlex.quote(c) for c in cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -454,6 +454,15 @@ int bm3d_init(int device) {
         for (int j = 0; j < 8; ++j)
             c8[k * 8 + j] = (float)(2.0 * std::cos(M_PI * (double)k * (double)(2 * j + 1) / (2.0 * 8.0)));
     HIP_TRY(upload_dct8(c8));
+    // z-axis coefficients for N = 1..32 (same expression and layout)
+    std::vector<float> cz;
+    for (int N = 1; N <= 32; ++N)
+        for (int k = 0; k < N; ++k)
+            for (int j = 0; j < N; ++j)
+                cz.push_back((float)(2.0 * std::cos(M_PI * (double)k * (double)(2 * j + 1) / (2.0 * (double)N))));
+    HIP_TRY(upload_dctz(cz.data(), 32));
+    if (!verify_dct_constants())
+        return set_error(BM3D_E_STATE, "bm3d_init: compiled DCT constants differ from this host's cos(); rebuild");
     g_dev.ready = true;
     return BM3D_OK;
 }

@@ -31,6 +31,7 @@ struct FilterArgs {
     const int* count;          // [nb]
     const int* pos;            // [nb * G] packed (y << 16) | x
     const int* frame;          // [nb * G] frame index (V-BM3D) or null
+    const int* order;          // processing order of the groups (bucketed by size) or null
     const float* const* src[3];  // per plane: device array of frame plane pointers
     const float* const* ref[3];
     const PlaneTables* tables;   // device array, one per plane
@@ -110,6 +111,11 @@ size_t match_wave_words(int max_cand, int B, int G);
 size_t match_lds_bytes(int max_cand, int B, int G);
 size_t filter_lds_bytes(int B, int G);
 hipError_t upload_dct8(const float* c64);
+hipError_t upload_dctz(const float* coef, int nmax);
+bool filter_fits(int B, int G, bool wiener);
+hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st);
+bool filter_uses_buckets(int B, int G);
+bool verify_dct_constants();
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
                         float* key, hipStream_t st);
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st);

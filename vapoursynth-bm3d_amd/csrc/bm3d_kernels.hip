@@ -8,11 +8,14 @@
 // with cross-lane ballots instead of shared-memory reductions.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
+#include <type_traits>
 
 #include "bm3d_params.h"
 #include "bm3d_device.h"
 #include "bm3d_kernels.h"
+#include "bm3d_dct_coefs.h"
 
 namespace bm3d {
 
@@ -513,6 +516,20 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
 // in registers with one lane per (y, x) column.  Every chain is the one
 // dct2_line / dct3_line evaluates, so results equal the generic path's.
 __constant__ float c_dct8[64];  // float(2 cos(pi k (2j+1) / 16)) at [k * 8 + j]
+// Length-N coefficients for N = 1..32 at c_dctz[zoff(N) + k * N + j]; constant
+// memory so that the wave-uniform reads of the z pass are scalar loads.
+constexpr int kZMax = 32;
+__constant__ float c_dctz[kZMax * (kZMax + 1) * (2 * kZMax + 1) / 6];
+__host__ __device__ constexpr int zoff(int N) { return (N - 1) * N * (2 * N - 1) / 6; }
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 __device__ __forceinline__ void dct8_fwd(const float* in, float* out) {
     float u[4], v[4];
@@ -521,28 +538,39 @@ __device__ __forceinline__ void dct8_fwd(const float* in, float* out) {
         u[j] = in[j] + in[7 - j];
         v[j] = in[j] - in[7 - j];
     }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    static_for<0, 8>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
         const float* w = (k & 1) ? v : u;
-        float acc = c_dct8[k * 8] * w[0];
-#pragma unroll
-        for (int j = 1; j < 4; ++j) acc = __builtin_fmaf(c_dct8[k * 8 + j], w[j], acc);
+        constexpr float c0 = DctC<8>::v[k * 8];
+        float acc = c0 * w[0];
+        static_for<1, 4>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            constexpr float c = DctC<8>::v[k * 8 + j];
+            acc = __builtin_fmaf(c, w[j], acc);
+        });
         out[k] = acc;
-    }
+    });
 }
 
 __device__ __forceinline__ void dct8_inv(const float* in, float* out) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    static_for<0, 4>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
         float e = in[0];
-#pragma unroll
-        for (int k = 2; k < 8; k += 2) e = __builtin_fmaf(c_dct8[k * 8 + j], in[k], e);
-        float o = c_dct8[8 + j] * in[1];
-#pragma unroll
-        for (int k = 3; k < 8; k += 2) o = __builtin_fmaf(c_dct8[k * 8 + j], in[k], o);
+        static_for<1, 4>([&](auto kc) {
+            constexpr int k = 2 * decltype(kc)::value;
+            constexpr float c = DctC<8>::v[k * 8 + j];
+            e = __builtin_fmaf(c, in[k], e);
+        });
+        constexpr float c1 = DctC<8>::v[8 + j];
+        float o = c1 * in[1];
+        static_for<1, 4>([&](auto kc) {
+            constexpr int k = 2 * decltype(kc)::value + 1;
+            constexpr float c = DctC<8>::v[k * 8 + j];
+            o = __builtin_fmaf(c, in[k], o);
+        });
         out[j] = e + o;
         out[7 - j] = e - o;
-    }
+    });
 }
 
 // x (rows) then y (columns) passes of the n slices in LDS (row stride 9).
@@ -587,75 +615,184 @@ __device__ __forceinline__ void passes_xy(float* g, int n) {
 }
 
 // Forward z transform of the lane's column (y, x): g[z*72 + off] -> X[k].
-template <int GMAX>
-__device__ __forceinline__ void zfwd(const float* g, int off, int n, const float* __restrict__ cz, float* X) {
-    const int h = n >> 1;
-    float u[GMAX / 2 > 0 ? GMAX / 2 : 1], v[GMAX / 2 > 0 ? GMAX / 2 : 1];
+// Forward z transform (length N, compile time) of the lane's column (y, x):
+// g[z*72 + off] -> X[k], with the chains of dct2_line.
+template <int N>
+__device__ __forceinline__ void zfwdN(const float* g, int off, float* X) {
+    constexpr int h = N / 2;
+    float u[h > 0 ? h : 1], v[h > 0 ? h : 1];
 #pragma unroll
-    for (int j = 0; j < GMAX / 2; ++j) {
-        if (j < h) {
-            float a0 = g[j * 72 + off], b0 = g[(n - 1 - j) * 72 + off];
-            u[j] = a0 + b0;
-            v[j] = a0 - b0;
-        }
+    for (int j = 0; j < h; ++j) {
+        const float a0 = g[j * 72 + off], b0 = g[(N - 1 - j) * 72 + off];
+        u[j] = a0 + b0;
+        v[j] = a0 - b0;
     }
-    const float mid = (n & 1) ? g[h * 72 + off] : 0.0f;
-#pragma unroll
-    for (int k = 0; k < GMAX; ++k) {
-        if (k < n) {
-            float acc;
-            if ((k & 1) == 0) {
-                if (h > 0) {
-                    acc = cz[k * n] * u[0];
-#pragma unroll
-                    for (int j = 1; j < GMAX / 2; ++j)
-                        if (j < h) acc = __builtin_fmaf(cz[k * n + j], u[j], acc);
-                    if (n & 1) acc = __builtin_fmaf(cz[k * n + h], mid, acc);
-                } else {
-                    acc = cz[0] * mid;
+    const float mid = (N & 1) ? g[h * 72 + off] : 0.0f;
+    static_for<0, N>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        float acc;
+        if constexpr ((k & 1) == 0) {
+            if constexpr (h > 0) {
+                constexpr float c0 = DctC<N>::v[k * N];
+                acc = c0 * u[0];
+                static_for<1, h>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    constexpr float c = DctC<N>::v[k * N + j];
+                    acc = __builtin_fmaf(c, u[j], acc);
+                });
+                if constexpr (N & 1) {
+                    constexpr float cm = DctC<N>::v[k * N + h];
+                    acc = __builtin_fmaf(cm, mid, acc);
                 }
             } else {
-                acc = cz[k * n] * v[0];
-#pragma unroll
-                for (int j = 1; j < GMAX / 2; ++j)
-                    if (j < h) acc = __builtin_fmaf(cz[k * n + j], v[j], acc);
+                constexpr float c0 = DctC<N>::v[0];
+                acc = c0 * mid;
             }
-            X[k] = acc;
+        } else {
+            constexpr float c0 = DctC<N>::v[k * N];
+            acc = c0 * v[0];
+            static_for<1, h>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                constexpr float c = DctC<N>::v[k * N + j];
+                acc = __builtin_fmaf(c, v[j], acc);
+            });
         }
+        X[k] = acc;
+    });
+}
+
+// Inverse z transform (chains of dct3_line) of X into the lane's column.
+template <int N>
+__device__ __forceinline__ void zinvN(float* g, int off, const float* X) {
+    constexpr int h = N / 2;
+    static_for<0, h + (N & 1)>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        float e = X[0];
+        static_for<1, (N + 1) / 2>([&](auto kc) {
+            constexpr int k = 2 * decltype(kc)::value;
+            constexpr float c = DctC<N>::v[k * N + j];
+            e = __builtin_fmaf(c, X[k], e);
+        });
+        if constexpr (j < h) {
+            constexpr float c1 = DctC<N>::v[N + j];
+            float o = c1 * X[1];
+            static_for<1, N / 2>([&](auto kc) {
+                constexpr int k = 2 * decltype(kc)::value + 1;
+                constexpr float c = DctC<N>::v[k * N + j];
+                o = __builtin_fmaf(c, X[k], o);
+            });
+            g[j * 72 + off] = e + o;
+            g[(N - 1 - j) * 72 + off] = e - o;
+        } else {
+            g[j * 72 + off] = e;
+        }
+    });
+}
+
+// One group of N members, every processed plane (N compile time).
+template <int N, bool WIENER>
+__device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int* zyx, float* gs, float* gr) {
+    const int lane = lane_id();
+    const int G = a.p.group;
+    const int y = lane >> 3, x = lane & 7;
+    const int off = y * 9 + x;
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!a.process[pl]) continue;
+        const PlaneTables& tb = a.tables[pl];
+        const float* const* sp = a.src[pl];
+        const float* const* rp = a.ref[pl];
+#pragma unroll
+        for (int z = 0; z < N; ++z) {
+            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
+            gs[z * 72 + off] = sp[pz][(size_t)(py + y) * a.p.pitch + px + x];
+            if (WIENER) gr[z * 72 + off] = rp[pz][(size_t)(py + y) * a.p.pitch + px + x];
+        }
+        wave_sync_lds();
+        float X[N];
+        float den_w;
+        if (!WIENER) {
+            float thr4[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) thr4[q] = tb.thr[N - 1][q];
+            passes_xy<true>(gs, N);
+            zfwdN<N>(gs, off, X);
+            int kept = 0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const int fl = (x == 0) + (y == 0) + (k == 0);
+                const float thr = fl == 0 ? thr4[0] : fl == 1 ? thr4[1] : fl == 2 ? thr4[2] : thr4[3];
+                const float s = X[k];
+                if (s > thr || s < -thr) ++kept;
+                else X[k] = 0.0f;
+            }
+            for (int o = 32; o >= 1; o >>= 1) kept += __shfl_xor(kept, o);
+            den_w = kept < 1 ? 1.0f : 1.0f / (float)kept;
+        } else {
+            passes_xy<true>(gr, N);
+            float R[N];
+            zfwdN<N>(gr, off, R);
+            passes_xy<true>(gs, N);
+            zfwdN<N>(gs, off, X);
+            const float sig2 = tb.wiener_sigma_sqr[N - 1];
+            // w^2 stored chain-major: SSE2 lane c = x % 4 takes the flat
+            // indices e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
+            float* w2 = gr;
+            const int c = x & 3;
+            const int ibase = y * 2 + (x >> 2);
+#pragma unroll
+            for (int k = 0; k < N; ++k) {
+                const float r2 = R[k] * R[k];
+                const float w = r2 * rcp_host(a.rcp_table, a.rcp_shift, r2 + sig2);
+                X[k] = X[k] * w;
+                w2[c * 16 * N + k * 16 + ibase] = w * w;
+            }
+            wave_sync_lds();
+            float part = 0.0f;
+            if (lane < 4) {
+                const float4* ch = reinterpret_cast<const float4*>(w2 + lane * 16 * N);
+#pragma unroll
+                for (int q0 = 0; q0 < 4 * N; q0 += 8) {
+                    float4 v4[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q0 + q < 4 * N) v4[q] = ch[q0 + q];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        if (q0 + q < 4 * N) {
+                            part = part + v4[q].x;
+                            part = part + v4[q].y;
+                            part = part + v4[q].z;
+                            part = part + v4[q].w;
+                        }
+                    }
+                }
+            }
+            const float p0 = __shfl(part, 0), p1 = __shfl(part, 1), p2 = __shfl(part, 2), p3 = __shfl(part, 3);
+            float l2 = 0.0f + (((p0 + p1) + p2) + p3);
+            l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;
+            den_w = 1.0f / l2;
+            wave_sync_lds();
+        }
+        zinvN<N>(gs, off, X);
+        wave_sync_lds();
+        passes_xy<false>(gs, N);
+        const float num_w = (float)((double)den_w / tb.final_amp[N - 1]);
+        float* contrib = a.contrib[pl] + (size_t)g_id * G * 64;
+#pragma unroll
+        for (int z = 0; z < N; ++z) contrib[z * 64 + lane] = gs[z * 72 + off] * num_w;
+        if (lane == 0) a.denw[pl][g_id] = den_w;
+        wave_sync_lds();
     }
 }
 
-// Inverse z transform of X[k] into the lane's column of g.
-template <int GMAX>
-__device__ __forceinline__ void zinv(float* g, int off, int n, const float* __restrict__ cz, const float* X) {
-    const int h = n >> 1;
-#pragma unroll
-    for (int j = 0; j < (GMAX + 1) / 2; ++j) {
-        if (j < h + (n & 1)) {
-            float e = X[0];
-#pragma unroll
-            for (int k = 2; k < GMAX; k += 2)
-                if (k < n) e = __builtin_fmaf(cz[k * n + j], X[k], e);
-            float o = 0.0f;
-            if (j < h) {
-                o = cz[n + j] * X[1];
-#pragma unroll
-                for (int k = 3; k < GMAX; k += 2)
-                    if (k < n) o = __builtin_fmaf(cz[k * n + j], X[k], o);
-                g[j * 72 + off] = e + o;
-                g[(n - 1 - j) * 72 + off] = e - o;
-            } else {
-                g[j * 72 + off] = e;
-            }
-        }
-    }
-}
-
+// One wave per group, taken from `order` (groups bucketed by size so that
+// neighbouring waves run the same specialisation).
 template <int GMAX, bool WIENER>
 __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
-    const int g_id = a.b0 + blockIdx.x;
-    if (g_id >= a.nb) return;
+    const int slot = a.b0 + blockIdx.x;
+    if (slot >= a.nb) return;
+    const int g_id = a.order ? a.order[slot] : slot;
     const int lane = lane_id();
     const int G = a.p.group;
     const int n = uniform(a.count[g_id] < G ? a.count[g_id] : G);
@@ -669,85 +806,59 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
         zyx[3 * z + 2] = pk & 0xffff;
     }
     wave_sync_lds();
-    const int y = lane >> 3, x = lane & 7;
-    const int off = y * 9 + x;
-    const float* cz = a.coef + a.coef_off[n];
-    for (int pl = 0; pl < 3; ++pl) {
-        if (!a.process[pl]) continue;
-        const PlaneTables& tb = a.tables[pl];
-        const float* const* sp = a.src[pl];
-        const float* const* rp = a.ref[pl];
-        for (int z = 0; z < n; ++z) {
-            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
-            gs[z * 72 + off] = sp[pz][(size_t)(py + y) * a.p.pitch + px + x];
-            if (WIENER) gr[z * 72 + off] = rp[pz][(size_t)(py + y) * a.p.pitch + px + x];
-        }
-        wave_sync_lds();
-        float X[GMAX];
-        float den_w;
-        if (!WIENER) {
-            passes_xy<true>(gs, n);
-            zfwd<GMAX>(gs, off, n, cz, X);
-            int kept = 0;
-#pragma unroll
-            for (int k = 0; k < GMAX; ++k) {
-                if (k < n) {
-                    const float thr = tb.thr[n - 1][(x == 0) + (y == 0) + (k == 0)];
-                    const float s = X[k];
-                    if (s > thr || s < -thr) ++kept;
-                    else X[k] = 0.0f;
-                }
-            }
-            for (int o = 32; o >= 1; o >>= 1) kept += __shfl_xor(kept, o);
-            den_w = kept < 1 ? 1.0f : 1.0f / (float)kept;
-        } else {
-            passes_xy<true>(gr, n);
-            float R[GMAX];
-            zfwd<GMAX>(gr, off, n, cz, R);
-            passes_xy<true>(gs, n);
-            zfwd<GMAX>(gs, off, n, cz, X);
-            const float sig2 = tb.wiener_sigma_sqr[n - 1];
-            // w^2 stored chain-major: SSE2 lane c = x % 4 takes flat indices
-            // e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
-            float* w2 = gr;
-            const int c = x & 3;
-            const int ibase = y * 2 + (x >> 2);
-#pragma unroll
-            for (int k = 0; k < GMAX; ++k) {
-                if (k < n) {
-                    const float r2 = R[k] * R[k];
-                    const float w = r2 * rcp_host(a.rcp_table, a.rcp_shift, r2 + sig2);
-                    X[k] = X[k] * w;
-                    w2[c * 16 * n + k * 16 + ibase] = w * w;
-                }
-            }
-            wave_sync_lds();
-            float part = 0.0f;
-            if (lane < 4) {
-                const float4* ch = reinterpret_cast<const float4*>(w2 + lane * 16 * n);
-                for (int q = 0; q < 4 * n; ++q) {
-                    float4 v4 = ch[q];
-                    part = part + v4.x;
-                    part = part + v4.y;
-                    part = part + v4.z;
-                    part = part + v4.w;
-                }
-            }
-            const float p0 = __shfl(part, 0), p1 = __shfl(part, 1), p2 = __shfl(part, 2), p3 = __shfl(part, 3);
-            float l2 = 0.0f + (((p0 + p1) + p2) + p3);
-            l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;
-            den_w = 1.0f / l2;
-            wave_sync_lds();
-        }
-        zinv<GMAX>(gs, off, n, cz, X);
-        wave_sync_lds();
-        passes_xy<false>(gs, n);
-        const float num_w = (float)((double)den_w / tb.final_amp[n - 1]);
-        float* contrib = a.contrib[pl] + (size_t)g_id * G * 64;
-        for (int z = 0; z < n; ++z) contrib[z * 64 + lane] = gs[z * 72 + off] * num_w;
-        if (lane == 0) a.denw[pl][g_id] = den_w;
-        wave_sync_lds();
+    switch (n) {
+#define BM3D_G8(NN)                                                         \
+    case NN:                                                                \
+        if constexpr (NN <= GMAX) group8<NN, WIENER>(a, g_id, zyx, gs, gr); \
+        break;
+        BM3D_G8(1) BM3D_G8(2) BM3D_G8(3) BM3D_G8(4) BM3D_G8(5) BM3D_G8(6) BM3D_G8(7) BM3D_G8(8)
+        BM3D_G8(9) BM3D_G8(10) BM3D_G8(11) BM3D_G8(12) BM3D_G8(13) BM3D_G8(14) BM3D_G8(15) BM3D_G8(16)
+        BM3D_G8(17) BM3D_G8(18) BM3D_G8(19) BM3D_G8(20) BM3D_G8(21) BM3D_G8(22) BM3D_G8(23) BM3D_G8(24)
+        BM3D_G8(25) BM3D_G8(26) BM3D_G8(27) BM3D_G8(28) BM3D_G8(29) BM3D_G8(30) BM3D_G8(31) BM3D_G8(32)
+#undef BM3D_G8
+    default:
+        break;
     }
+}
+
+// Bucketing of groups by size: histogram, exclusive scan (largest n first),
+// scatter.  Counts are integers, so the buckets are deterministic; the
+// order inside a bucket does not affect any result.
+__global__ void k_zero_ints(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_size_hist(const int* __restrict__ count, int nb, int G, int* hist) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int n = count[b] < G ? count[b] : G;
+    atomicAdd(&hist[n], 1);
+}
+
+__global__ void k_size_scan(int* hist, int G) {
+    if (threadIdx.x != 0) return;
+    int acc = 0;
+    for (int n = G; n >= 1; --n) {
+        int c = hist[n];
+        hist[n] = acc;
+        acc += c;
+    }
+}
+
+__global__ void k_size_scatter(const int* __restrict__ count, int nb, int G, int* cursor, int* order) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    int n = count[b] < G ? count[b] : G;
+    order[atomicAdd(&cursor[n], 1)] = b;
+}
+
+hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_ints, dim3((G + 1 + 255) / 256), dim3(256), 0, st, hist, G + 1);
+    hipLaunchKernelGGL(k_size_hist, dim3((nb + 255) / 256), dim3(256), 0, st, count, nb, G, hist);
+    hipLaunchKernelGGL(k_size_scan, dim3(1), dim3(64), 0, st, hist, G);
+    hipLaunchKernelGGL(k_size_scatter, dim3((nb + 255) / 256), dim3(256), 0, st, count, nb, G, hist, order);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -985,7 +1096,34 @@ size_t filter8_lds_bytes(int G, bool wiener) {
     return sizeof(float) * (size_t)gm * 72 * (wiener ? 2 : 1) + sizeof(int) * 3 * (size_t)gm;
 }
 
+// The kernels use the generated coefficient constants; they must equal what
+// the host's std::cos gives (the tables of upload_coefs and the oracle).
+template <int N>
+static bool verify_dctc() {
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j) {
+            float want = (float)(2.0 * std::cos(M_PI * (double)k * (double)(2 * j + 1) / (2.0 * (double)N)));
+            if (__builtin_memcmp(&want, &DctC<N>::v[k * N + j], 4) != 0) return false;
+        }
+    if constexpr (N > 1) return verify_dctc<N - 1>();
+    return true;
+}
+
+bool verify_dct_constants() { return verify_dctc<kDctCMax>(); }
+
+bool filter_uses_buckets(int B, int G) { return B == 8 && filter8_gmax(G) > 0; }
+
+bool filter_fits(int B, int G, bool wiener) {
+    if (B == 8 && filter8_gmax(G) > 0) return filter8_lds_bytes(G, wiener) <= 65536;
+    return filter_lds_bytes(B, G) <= 65536;
+}
+
 hipError_t upload_dct8(const float* c64) { return hipMemcpyToSymbol(HIP_SYMBOL(c_dct8), c64, 64 * sizeof(float)); }
+
+hipError_t upload_dctz(const float* coef, int nmax) {
+    // host table layout: [off(N) + k*N + j] with off(N) = sum_{M<N} M^2
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_dctz), coef, sizeof(float) * zoff(nmax + 1));
+}
 
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
     if (a.nb == 0) return hipSuccess;

@@ -413,8 +413,16 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     fa.rcp_shift = 23 - rcp_bits();
     fa.coef = f->coef_dev.as<float>();
     fa.coef_off = f->coef_off_dev.as<int>();
+    if (!filter_fits(B, G, f->wiener))
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: block_size/group_size too large for the on-chip group buffer");
     if (f->wiener && rcp_bits() == 0) fa.rcp_shift = 23;
     prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
+    if (filter_uses_buckets(B, G)) {
+        if ((rc = f->m_order.ensure(sizeof(int) * (size_t)nb))) return rc;
+        if ((rc = f->m_hist.ensure(sizeof(int) * (size_t)(G + 1)))) return rc;
+        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_hist.as<int>(), f->m_order.as<int>(), st));
+        fa.order = f->m_order.as<int>();
+    }
     HIP_TRY(launch_filter(fa, st));
     prof_end(f, kTFilter);
 
@@ -640,7 +648,15 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     fa.rcp_shift = rcp_bits() == 0 ? 23 : 23 - rcp_bits();
     fa.coef = f->coef_dev.as<float>();
     fa.coef_off = f->coef_off_dev.as<int>();
+    if (!filter_fits(B, G, f->wiener))
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: block_size/group_size too large for the on-chip group buffer");
     prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
+    if (filter_uses_buckets(B, G)) {
+        if ((rc = f->m_order.ensure(sizeof(int) * (size_t)nb))) return rc;
+        if ((rc = f->m_hist.ensure(sizeof(int) * (size_t)(G + 1)))) return rc;
+        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_hist.as<int>(), f->m_order.as<int>(), st));
+        fa.order = f->m_order.as<int>();
+    }
     HIP_TRY(launch_filter(fa, st));
     prof_end(f, kTFilter);
     // --- aggregation into the stack
