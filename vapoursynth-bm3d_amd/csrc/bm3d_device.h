@@ -255,4 +255,129 @@ __device__ __forceinline__ float rcp_host(const float* __restrict__ tbl, int shi
     return (u >> 31) ? -r : r;
 }
 
+// ---------------------------------------------------------------------------
+// ordered gather aggregation shared by the spatial and temporal kernels
+// ---------------------------------------------------------------------------
+
+constexpr int kAggCap = 448;  // records buffered per wave (LDS)
+
+// Adds the buffered records [0, nrec) in order.  Loads of 8 records are
+// issued before any of them is added, so their latencies overlap; the adds
+// themselves keep the record order.
+__device__ __forceinline__ void agg_consume(const int* rq, const int* rbase, const int* rbi, int nrec, int B,
+                                            int px, int py, bool valid, const bool* proc,
+                                            const float* __restrict__ c0, const float* __restrict__ c1,
+                                            const float* __restrict__ c2, const float* __restrict__ d0,
+                                            const float* __restrict__ d1, const float* __restrict__ d2,
+                                            float* num, float* den) {
+    const float* cp[3] = {c0, c1, c2};
+    const float* dp[3] = {d0, d1, d2};
+    const int BB = B * B;
+    for (int r0 = 0; r0 < nrec; r0 += 8) {
+        float v[3][8], w[3][8];
+        bool cov[8];
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            const int r = r0 + rr;
+            cov[rr] = false;
+            if (r < nrec) {
+                const int q = rq[r];
+                const int dy = py - (q >> 16), dx = px - (q & 0xffff);
+                cov[rr] = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
+                const size_t off = (size_t)rbase[r] * BB + (size_t)(dy * B + dx);
+                const int bi = rbi[r];
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl) {
+                    if (!proc[pl]) continue;
+                    v[pl][rr] = cov[rr] ? cp[pl][off] : 0.0f;
+                    w[pl][rr] = dp[pl][bi];
+                }
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < 8; ++rr) {
+            if (r0 + rr < nrec && cov[rr]) {
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl) {
+                    if (!proc[pl]) continue;
+                    num[pl] = num[pl] + v[pl][rr];
+                    den[pl] = den[pl] + w[pl][rr];
+                }
+            }
+        }
+    }
+}
+
+// Walks the reference blocks of the rectangle [a_lo, a_hi] x [c_lo, c_hi] in
+// scan order and accumulates every member (of frame fr, when frame != null)
+// that overlaps the 8x8 tile at (y0, x0), in (block, member) order.
+__device__ inline void agg_tile(const StageParams& p, const int* __restrict__ count, const int* __restrict__ pos,
+                         const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi, int y0,
+                         int x0, int px, int py, bool valid, const bool* proc, const float* const* contrib,
+                         const float* const* denw, int* rq, int* rbase, int* rbi, float* num, float* den) {
+    const int lane = lane_id();
+    const int B = p.block, G = p.group;
+    const int gp = G <= 1 ? 1 : G <= 2 ? 2 : G <= 4 ? 4 : G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 64;
+    const int per = 64 / gp;  // reference blocks per wave-load
+    int nrec = 0;
+    for (int ar = a_lo; ar <= a_hi; ++ar) {
+        for (int ac0 = c_lo; ac0 <= c_hi; ac0 += per) {
+            for (int z0 = 0; z0 < G; z0 += 64) {
+                const int ac = ac0 + lane / gp;
+                const int z = z0 + lane % gp;
+                const int b = ar * p.nx + ac;
+                bool ov = false;
+                int pk = 0;
+                if (ac <= c_hi && z < G) {
+                    const int n = count[b];
+                    if (z < n) {
+                        pk = pos[(size_t)b * G + z];
+                        const int qy = pk >> 16, qx = pk & 0xffff;
+                        ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
+                        if (frame) ov = ov && frame[(size_t)b * G + z] == fr;
+                    }
+                }
+                const uint64_t m = __ballot(ov);
+                const int cnt = __popcll(m);
+                if (cnt == 0) continue;
+                if (nrec + cnt > kAggCap) {
+                    agg_consume(rq, rbase, rbi, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2],
+                                denw[0], denw[1], denw[2], num, den);
+                    wave_sync_lds();
+                    nrec = 0;
+                }
+                const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (ov) {
+                    rq[at] = pk;
+                    rbase[at] = b * G + z;
+                    rbi[at] = b;
+                }
+                nrec += cnt;
+                if (gp == 64 && G <= 64) break;
+            }
+        }
+    }
+    wave_sync_lds();
+    agg_consume(rq, rbase, rbi, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], denw[0], denw[1],
+                denw[2], num, den);
+}
+
+// Conservative rectangle of reference blocks whose members can reach a tile
+// when they lie at most `reach` from their reference block.
+__device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0, int reach, int& a_lo, int& a_hi,
+                                           int& c_lo, int& c_hi) {
+    const int B = p.block, s = p.block_step;
+    const int lo_y = y0 - B + 1 - reach, hi_y = y0 + 7 + reach;
+    const int lo_x = x0 - B + 1 - reach, hi_x = x0 + 7 + reach;
+    a_lo = lo_y > 0 ? lo_y / s - 1 : 0;
+    a_hi = hi_y / s + 1;
+    c_lo = lo_x > 0 ? lo_x / s - 1 : 0;
+    c_hi = hi_x / s + 1;
+    a_lo = a_lo < 0 ? 0 : a_lo;
+    c_lo = c_lo < 0 ? 0 : c_lo;
+    a_hi = a_hi > p.ny - 1 ? p.ny - 1 : a_hi;
+    c_hi = c_hi > p.nx - 1 ? p.nx - 1 : c_hi;
+}
+
 }  // namespace bm3d

@@ -690,8 +690,29 @@ __device__ __forceinline__ void zinvN(float* g, int off, const float* X) {
 }
 
 // One group of N members, every processed plane (N compile time).
+// Loads the N member blocks of one plane into g[z][y][x] (row stride 9).
+template <int N>
+__device__ __forceinline__ void gather8(float* g, const float* const* planes, const int* zyx, int pitch, int off,
+                                        int y, int x, bool single_frame) {
+    if (single_frame) {
+        const float* base = planes[0];
+#pragma unroll
+        for (int z = 0; z < N; ++z) {
+            const int py = zyx[3 * z + 1], px = zyx[3 * z + 2];
+            g[z * 72 + off] = base[(size_t)(py + y) * pitch + px + x];
+        }
+    } else {
+#pragma unroll
+        for (int z = 0; z < N; ++z) {
+            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
+            g[z * 72 + off] = planes[pz][(size_t)(py + y) * pitch + px + x];
+        }
+    }
+    wave_sync_lds();
+}
+
 template <int N, bool WIENER>
-__device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int* zyx, float* gs, float* gr) {
+__device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int* zyx, float* gs) {
     const int lane = lane_id();
     const int G = a.p.group;
     const int y = lane >> 3, x = lane & 7;
@@ -699,18 +720,10 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
     for (int pl = 0; pl < 3; ++pl) {
         if (!a.process[pl]) continue;
         const PlaneTables& tb = a.tables[pl];
-        const float* const* sp = a.src[pl];
-        const float* const* rp = a.ref[pl];
-#pragma unroll
-        for (int z = 0; z < N; ++z) {
-            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
-            gs[z * 72 + off] = sp[pz][(size_t)(py + y) * a.p.pitch + px + x];
-            if (WIENER) gr[z * 72 + off] = rp[pz][(size_t)(py + y) * a.p.pitch + px + x];
-        }
-        wave_sync_lds();
         float X[N];
         float den_w;
         if (!WIENER) {
+            gather8<N>(gs, a.src[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
             float thr4[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) thr4[q] = tb.thr[N - 1][q];
@@ -728,15 +741,21 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
             for (int o = 32; o >= 1; o >>= 1) kept += __shfl_xor(kept, o);
             den_w = kept < 1 ? 1.0f : 1.0f / (float)kept;
         } else {
-            passes_xy<true>(gr, N);
+            // one LDS buffer: the reference group's transform goes to registers
+            // first, then the source group reuses the buffer
+            gather8<N>(gs, a.ref[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
+            passes_xy<true>(gs, N);
             float R[N];
-            zfwdN<N>(gr, off, R);
+            zfwdN<N>(gs, off, R);
+            wave_sync_lds();
+            gather8<N>(gs, a.src[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
             passes_xy<true>(gs, N);
             zfwdN<N>(gs, off, X);
+            wave_sync_lds();
             const float sig2 = tb.wiener_sigma_sqr[N - 1];
             // w^2 stored chain-major: SSE2 lane c = x % 4 takes the flat
             // indices e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
-            float* w2 = gr;
+            float* w2 = gs;
             const int c = x & 3;
             const int ibase = y * 2 + (x >> 2);
 #pragma unroll
@@ -796,9 +815,8 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     const int lane = lane_id();
     const int G = a.p.group;
     const int n = uniform(a.count[g_id] < G ? a.count[g_id] : G);
-    float* gs = fsm;                       // [GMAX][8][9]
-    float* gr = gs + GMAX * 72;            // [GMAX][8][9] (Wiener), then w^2 chains
-    int* zyx = reinterpret_cast<int*>(gr + (WIENER ? GMAX * 72 : 0));
+    float* gs = fsm;                       // [GMAX][8][9], later the w^2 chains
+    int* zyx = reinterpret_cast<int*>(gs + GMAX * 72);
     for (int z = lane; z < n; z += 64) {
         int pk = a.pos[(size_t)g_id * G + z];
         zyx[3 * z] = a.frame ? a.frame[(size_t)g_id * G + z] : 0;
@@ -809,7 +827,7 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     switch (n) {
 #define BM3D_G8(NN)                                                         \
     case NN:                                                                \
-        if constexpr (NN <= GMAX) group8<NN, WIENER>(a, g_id, zyx, gs, gr); \
+        if constexpr (NN <= GMAX) group8<NN, WIENER>(a, g_id, zyx, gs);     \
         break;
         BM3D_G8(1) BM3D_G8(2) BM3D_G8(3) BM3D_G8(4) BM3D_G8(5) BM3D_G8(6) BM3D_G8(7) BM3D_G8(8)
         BM3D_G8(9) BM3D_G8(10) BM3D_G8(11) BM3D_G8(12) BM3D_G8(13) BM3D_G8(14) BM3D_G8(15) BM3D_G8(16)
@@ -902,63 +920,24 @@ __device__ void write_output(const OutputSpec& o, int x, int y, const float* v, 
 }
 
 __global__ __launch_bounds__(256) void k_aggregate(AggArgs a) {
+    __shared__ int srec[4][3 * kAggCap];
     const int lane = lane_id();
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6;
+    const int tile = blockIdx.x * 4 + wave;
     const int tiles_x = (a.p.width + 7) >> 3;
     const int tiles_y = (a.p.height + 7) >> 3;
     if (tile >= tiles_x * tiles_y) return;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int B = a.p.block, G = a.p.group, BB = B * B;
     const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
     const bool valid = px < a.p.width && py < a.p.height;
     const int y0 = ty * 8, x0 = tx * 8;
-    // reference blocks whose search window can reach the tile (conservative)
-    const int lo_y = y0 - B + 1 - a.reach, hi_y = y0 + 7 + a.reach;
-    const int lo_x = x0 - B + 1 - a.reach, hi_x = x0 + 7 + a.reach;
-    const int s = a.p.block_step;
-    int a_lo = lo_y > 0 ? lo_y / s - 1 : 0, a_hi = hi_y / s + 1;
-    int c_lo = lo_x > 0 ? lo_x / s - 1 : 0, c_hi = hi_x / s + 1;
-    a_lo = a_lo < 0 ? 0 : a_lo;
-    c_lo = c_lo < 0 ? 0 : c_lo;
-    a_hi = a_hi > a.p.ny - 1 ? a.p.ny - 1 : a_hi;
-    c_hi = c_hi > a.p.nx - 1 ? a.p.nx - 1 : c_hi;
+    int a_lo, a_hi, c_lo, c_hi;
+    reach_rect(a.p, y0, x0, a.reach, a_lo, a_hi, c_lo, c_hi);
     bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
     float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
-    for (int ar = a_lo; ar <= a_hi; ++ar) {
-        for (int ac = c_lo; ac <= c_hi; ++ac) {
-            const int b = ar * a.p.nx + ac;
-            int n = a.count[b];
-            n = uniform(n < G ? n : G);
-            for (int z0 = 0; z0 < n; z0 += 64) {
-                int z = z0 + lane;
-                int pk = 0;
-                bool ov = false;
-                if (z < n) {
-                    pk = a.pos[(size_t)b * G + z];
-                    int qy = pk >> 16, qx = pk & 0xffff;
-                    ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
-                }
-                uint64_t m = __ballot(ov);
-                while (m) {
-                    int zl = __builtin_ctzll(m);
-                    m &= m - 1;
-                    int q = __builtin_amdgcn_readlane(pk, zl);
-                    int qy = q >> 16, qx = q & 0xffff;
-                    int dy = py - qy, dx = px - qx;
-                    bool cov = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
-                    size_t off = ((size_t)b * G + z0 + zl) * BB + (size_t)(dy * B + dx);
-#pragma unroll
-                    for (int pl = 0; pl < 3; ++pl) {
-                        if (!proc[pl]) continue;
-                        if (cov) {
-                            num[pl] = num[pl] + a.contrib[pl][off];
-                            den[pl] = den[pl] + a.denw[pl][b];
-                        }
-                    }
-                }
-            }
-        }
-    }
+    int* rq = srec[wave];
+    agg_tile(a.p, a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
+             a.denw, rq, rq + kAggCap, rq + 2 * kAggCap, num, den);
     if (!valid) return;
     float v[3];
 #pragma unroll
@@ -1093,7 +1072,7 @@ static int filter8_gmax(int G) { return G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32
 
 size_t filter8_lds_bytes(int G, bool wiener) {
     int gm = filter8_gmax(G);
-    return sizeof(float) * (size_t)gm * 72 * (wiener ? 2 : 1) + sizeof(int) * 3 * (size_t)gm;
+    return sizeof(float) * (size_t)gm * 72 + sizeof(int) * 3 * (size_t)gm;
 }
 
 // The kernels use the generated coefficient constants; they must equal what

@@ -265,64 +265,27 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 // +1 (den) of the output stack (VBM3D_Base.h:284-292).
 
 __global__ __launch_bounds__(256) void k_vstack(VStackArgs a) {
+    __shared__ int srec[4][3 * kAggCap];
     const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
     const int tiles_x = (a.p.width + 7) >> 3, tiles_y = (a.p.height + 7) >> 3;
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wid = blockIdx.x * 4 + wave;
     if (wid >= tiles_x * tiles_y * a.nframes) return;
     const int fr = wid / (tiles_x * tiles_y);
     const int tile = wid - fr * tiles_x * tiles_y;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int B = a.p.block, G = a.p.group, BB = B * B;
     const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
     const bool valid = px < a.p.width && py < a.p.height;
     const int y0 = ty * 8, x0 = tx * 8;
     const int dist = fr > a.cur ? fr - a.cur : a.cur - fr;
     const int reach = a.p.range + dist * (a.p.ps_range / a.p.ps_step * a.p.ps_step);
-    const int lo_y = y0 - B + 1 - reach, hi_y = y0 + 7 + reach;
-    const int lo_x = x0 - B + 1 - reach, hi_x = x0 + 7 + reach;
-    const int s = a.p.block_step;
-    int a_lo = lo_y > 0 ? lo_y / s - 1 : 0, a_hi = hi_y / s + 1;
-    int c_lo = lo_x > 0 ? lo_x / s - 1 : 0, c_hi = hi_x / s + 1;
-    a_lo = a_lo < 0 ? 0 : a_lo;
-    c_lo = c_lo < 0 ? 0 : c_lo;
-    a_hi = a_hi > a.p.ny - 1 ? a.p.ny - 1 : a_hi;
-    c_hi = c_hi > a.p.nx - 1 ? a.p.nx - 1 : c_hi;
+    int a_lo, a_hi, c_lo, c_hi;
+    reach_rect(a.p, y0, x0, reach, a_lo, a_hi, c_lo, c_hi);
     bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
     float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
-    for (int ar = a_lo; ar <= a_hi; ++ar) {
-        for (int ac = c_lo; ac <= c_hi; ++ac) {
-            const int b = ar * a.p.nx + ac;
-            int n = a.count[b];
-            n = uniform(n < G ? n : G);
-            for (int z0 = 0; z0 < n; z0 += 64) {
-                int z = z0 + lane;
-                int pk = 0;
-                bool ov = false;
-                if (z < n && a.frame[(size_t)b * G + z] == fr) {
-                    pk = a.pos[(size_t)b * G + z];
-                    int qy = pk >> 16, qx = pk & 0xffff;
-                    ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
-                }
-                uint64_t m = __ballot(ov);
-                while (m) {
-                    int zl = __builtin_ctzll(m);
-                    m &= m - 1;
-                    int q = __builtin_amdgcn_readlane(pk, zl);
-                    int dy = py - (q >> 16), dx = px - (q & 0xffff);
-                    bool cov = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
-                    size_t off = ((size_t)b * G + z0 + zl) * BB + (size_t)(dy * B + dx);
-#pragma unroll
-                    for (int pl = 0; pl < 3; ++pl) {
-                        if (!proc[pl]) continue;
-                        if (cov) {
-                            num[pl] = num[pl] + a.contrib[pl][off];
-                            den[pl] = den[pl] + a.denw[pl][b];
-                        }
-                    }
-                }
-            }
-        }
-    }
+    int* rq = srec[wave];
+    agg_tile(a.p, a.count, a.pos, a.frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
+             a.denw, rq, rq + kAggCap, rq + 2 * kAggCap, num, den);
     if (!valid) return;
     const int slot = 2 * (a.p.radius + fr - a.cur);
 #pragma unroll
