@@ -259,19 +259,17 @@ __device__ __forceinline__ float rcp_host(const float* __restrict__ tbl, int shi
 // ordered gather aggregation shared by the spatial and temporal kernels
 // ---------------------------------------------------------------------------
 
-constexpr int kAggCap = 448;  // records buffered per wave (LDS)
+constexpr int kAggCap = 256;   // records buffered per wave (LDS)
+constexpr int kAggWords = 5;   // per record: position, contribution base, den weight x 3
 
 // Adds the buffered records [0, nrec) in order.  Loads of 8 records are
 // issued before any of them is added, so their latencies overlap; the adds
 // themselves keep the record order.
-__device__ __forceinline__ void agg_consume(const int* rq, const int* rbase, const int* rbi, int nrec, int B,
-                                            int px, int py, bool valid, const bool* proc,
-                                            const float* __restrict__ c0, const float* __restrict__ c1,
-                                            const float* __restrict__ c2, const float* __restrict__ d0,
-                                            const float* __restrict__ d1, const float* __restrict__ d2,
-                                            float* num, float* den) {
+__device__ __forceinline__ void agg_consume(const int* rec, int nrec, int B, int px, int py, bool valid,
+                                            const bool* proc, const float* __restrict__ c0,
+                                            const float* __restrict__ c1, const float* __restrict__ c2, float* num,
+                                            float* den) {
     const float* cp[3] = {c0, c1, c2};
-    const float* dp[3] = {d0, d1, d2};
     const int BB = B * B;
     for (int r0 = 0; r0 < nrec; r0 += 8) {
         float v[3][8], w[3][8];
@@ -281,16 +279,16 @@ __device__ __forceinline__ void agg_consume(const int* rq, const int* rbase, con
             const int r = r0 + rr;
             cov[rr] = false;
             if (r < nrec) {
-                const int q = rq[r];
+                const int* e = rec + r * kAggWords;
+                const int q = e[0];
                 const int dy = py - (q >> 16), dx = px - (q & 0xffff);
                 cov[rr] = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
-                const size_t off = (size_t)rbase[r] * BB + (size_t)(dy * B + dx);
-                const int bi = rbi[r];
+                const size_t off = (size_t)e[1] * BB + (size_t)(dy * B + dx);
 #pragma unroll
                 for (int pl = 0; pl < 3; ++pl) {
                     if (!proc[pl]) continue;
                     v[pl][rr] = cov[rr] ? cp[pl][off] : 0.0f;
-                    w[pl][rr] = dp[pl][bi];
+                    w[pl][rr] = __int_as_float(e[2 + pl]);
                 }
             }
         }
@@ -312,55 +310,61 @@ __device__ __forceinline__ void agg_consume(const int* rq, const int* rbase, con
 // scan order and accumulates every member (of frame fr, when frame != null)
 // that overlaps the 8x8 tile at (y0, x0), in (block, member) order.
 __device__ inline void agg_tile(const StageParams& p, const int* __restrict__ count, const int* __restrict__ pos,
-                         const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi, int y0,
-                         int x0, int px, int py, bool valid, const bool* proc, const float* const* contrib,
-                         const float* const* denw, int* rq, int* rbase, int* rbi, float* num, float* den) {
+                                const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi,
+                                int y0, int x0, int px, int py, bool valid, const bool* proc,
+                                const float* const* contrib, const float* const* denw, int* rec, float* num,
+                                float* den) {
     const int lane = lane_id();
     const int B = p.block, G = p.group;
     const int gp = G <= 1 ? 1 : G <= 2 ? 2 : G <= 4 ? 4 : G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 64;
     const int per = 64 / gp;  // reference blocks per wave-load
+    const int ncol = c_hi - c_lo + 1;
+    const int chunks_per_row = (ncol + per - 1) / per;
+    const int zchunks = (G + 63) / 64;
+    const int total = (a_hi - a_lo + 1) * chunks_per_row * zchunks;
     int nrec = 0;
-    for (int ar = a_lo; ar <= a_hi; ++ar) {
-        for (int ac0 = c_lo; ac0 <= c_hi; ac0 += per) {
-            for (int z0 = 0; z0 < G; z0 += 64) {
-                const int ac = ac0 + lane / gp;
-                const int z = z0 + lane % gp;
-                const int b = ar * p.nx + ac;
-                bool ov = false;
-                int pk = 0;
-                if (ac <= c_hi && z < G) {
-                    const int n = count[b];
-                    if (z < n) {
-                        pk = pos[(size_t)b * G + z];
-                        const int qy = pk >> 16, qx = pk & 0xffff;
-                        ov = qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
-                        if (frame) ov = ov && frame[(size_t)b * G + z] == fr;
-                    }
-                }
-                const uint64_t m = __ballot(ov);
-                const int cnt = __popcll(m);
-                if (cnt == 0) continue;
-                if (nrec + cnt > kAggCap) {
-                    agg_consume(rq, rbase, rbi, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2],
-                                denw[0], denw[1], denw[2], num, den);
-                    wave_sync_lds();
-                    nrec = 0;
-                }
-                const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                if (ov) {
-                    rq[at] = pk;
-                    rbase[at] = b * G + z;
-                    rbi[at] = b;
-                }
-                nrec += cnt;
-                if (gp == 64 && G <= 64) break;
-            }
+    for (int it = 0; it < total; ++it) {
+        const int zc = it % zchunks;
+        const int rc = it / zchunks;
+        const int ar = a_lo + rc / chunks_per_row;
+        const int ac = c_lo + (rc % chunks_per_row) * per + lane / gp;
+        const int z = zc * 64 + lane % gp;
+        bool ov = false;
+        int pk = 0, b = 0;
+        float dw[3] = {0.0f, 0.0f, 0.0f};
+        if (ac <= c_hi && z < G) {
+            b = ar * p.nx + ac;
+            const int n = count[b];
+            pk = pos[(size_t)b * G + z];
+            const int fz = frame ? frame[(size_t)b * G + z] : fr;
+#pragma unroll
+            for (int pl = 0; pl < 3; ++pl)
+                if (proc[pl]) dw[pl] = denw[pl][b];
+            const int qy = pk >> 16, qx = pk & 0xffff;
+            ov = z < n && fz == fr && qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
         }
+        const uint64_t m = __ballot(ov);
+        const int cnt = __popcll(m);
+        if (cnt == 0) continue;
+        if (nrec + cnt > kAggCap) {
+            wave_sync_lds();
+            agg_consume(rec, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], num, den);
+            wave_sync_lds();
+            nrec = 0;
+        }
+        const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (ov) {
+            int* e = rec + at * kAggWords;
+            e[0] = pk;
+            e[1] = b * G + z;
+            e[2] = __float_as_int(dw[0]);
+            e[3] = __float_as_int(dw[1]);
+            e[4] = __float_as_int(dw[2]);
+        }
+        nrec += cnt;
     }
     wave_sync_lds();
-    agg_consume(rq, rbase, rbi, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], denw[0], denw[1],
-                denw[2], num, den);
+    agg_consume(rec, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], num, den);
 }
 
 // Conservative rectangle of reference blocks whose members can reach a tile

@@ -847,13 +847,27 @@ __global__ void k_zero_ints(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-__global__ void k_size_hist(const int* __restrict__ count, int nb, int G, int* hist) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    int n = count[b] < G ? count[b] : G;
-    atomicAdd(&hist[n], 1);
+// Group-size bucketing.  Each workgroup histograms a chunk of the match
+// counts in LDS and touches the global counters once per non-empty size, so
+// the global atomics do not serialise on the few popular sizes.
+constexpr int kBucketThreads = 256;
+constexpr int kBucketPerThread = 16;
+constexpr int kBucketChunk = kBucketThreads * kBucketPerThread;
+
+__global__ __launch_bounds__(kBucketThreads) void k_size_hist(const int* __restrict__ count, int nb, int G,
+                                                              int* hist) {
+    __shared__ int h[kMaxGroup + 1];
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
+    __syncthreads();
+    const int lo = blockIdx.x * kBucketChunk;
+    const int hi = min(nb, lo + kBucketChunk);
+    for (int b = lo + threadIdx.x; b < hi; b += kBucketThreads) atomicAdd(&h[min(count[b], G)], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
+// Exclusive prefix over sizes G..1 (largest groups first), in place.
 __global__ void k_size_scan(int* hist, int G) {
     if (threadIdx.x != 0) return;
     int acc = 0;
@@ -864,18 +878,39 @@ __global__ void k_size_scan(int* hist, int G) {
     }
 }
 
-__global__ void k_size_scatter(const int* __restrict__ count, int nb, int G, int* cursor, int* order) {
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    int n = count[b] < G ? count[b] : G;
-    order[atomicAdd(&cursor[n], 1)] = b;
+// Places every reference block at its size's cursor: ranks inside the chunk
+// come from LDS counters, and one global atomic per size reserves the
+// chunk's range.  The order inside a size is not deterministic; it does not
+// affect any result, since every group writes only its own contributions.
+__global__ __launch_bounds__(kBucketThreads) void k_size_scatter(const int* __restrict__ count, int nb, int G,
+                                                                 int* cursor, int* order) {
+    __shared__ int h[kMaxGroup + 1];
+    __shared__ int base[kMaxGroup + 1];
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
+    __syncthreads();
+    const int lo = blockIdx.x * kBucketChunk;
+    const int hi = min(nb, lo + kBucketChunk);
+    int rank[kBucketPerThread], size[kBucketPerThread];
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k) {
+        const int b = lo + threadIdx.x + k * kBucketThreads;
+        size[k] = b < hi ? min(count[b], G) : -1;
+        rank[k] = size[k] >= 0 ? atomicAdd(&h[size[k]], 1) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) base[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k)
+        if (size[k] >= 1) order[base[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
 }
 
 hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_ints, dim3((G + 1 + 255) / 256), dim3(256), 0, st, hist, G + 1);
-    hipLaunchKernelGGL(k_size_hist, dim3((nb + 255) / 256), dim3(256), 0, st, count, nb, G, hist);
+    const int chunks = (nb + kBucketChunk - 1) / kBucketChunk;
+    hipLaunchKernelGGL(k_size_hist, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist);
     hipLaunchKernelGGL(k_size_scan, dim3(1), dim3(64), 0, st, hist, G);
-    hipLaunchKernelGGL(k_size_scatter, dim3((nb + 255) / 256), dim3(256), 0, st, count, nb, G, hist, order);
+    hipLaunchKernelGGL(k_size_scatter, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist, order);
     return hipGetLastError();
 }
 
@@ -920,7 +955,7 @@ __device__ void write_output(const OutputSpec& o, int x, int y, const float* v, 
 }
 
 __global__ __launch_bounds__(256) void k_aggregate(AggArgs a) {
-    __shared__ int srec[4][3 * kAggCap];
+    __shared__ int srec[4][kAggWords * kAggCap];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int tile = blockIdx.x * 4 + wave;
@@ -937,7 +972,7 @@ __global__ __launch_bounds__(256) void k_aggregate(AggArgs a) {
     float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
     int* rq = srec[wave];
     agg_tile(a.p, a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
-             a.denw, rq, rq + kAggCap, rq + 2 * kAggCap, num, den);
+             a.denw, rq, num, den);
     if (!valid) return;
     float v[3];
 #pragma unroll

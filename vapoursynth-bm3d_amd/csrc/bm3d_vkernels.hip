@@ -265,7 +265,7 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 // +1 (den) of the output stack (VBM3D_Base.h:284-292).
 
 __global__ __launch_bounds__(256) void k_vstack(VStackArgs a) {
-    __shared__ int srec[4][3 * kAggCap];
+    __shared__ int srec[4][kAggWords * kAggCap];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (a.p.width + 7) >> 3, tiles_y = (a.p.height + 7) >> 3;
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(256) void k_vstack(VStackArgs a) {
     float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
     int* rq = srec[wave];
     agg_tile(a.p, a.count, a.pos, a.frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
-             a.denw, rq, rq + kAggCap, rq + 2 * kAggCap, num, den);
+             a.denw, rq, num, den);
     if (!valid) return;
     const int slot = 2 * (a.p.radius + fr - a.cur);
 #pragma unroll
