@@ -232,6 +232,12 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Load from global memory (the pointer may have lost its address space
+// through a select; a flat load would also count against the LDS counter).
+__device__ __forceinline__ float ld_global(const float* p) {
+    return *(const __attribute__((address_space(1))) float*)p;
+}
+
 __device__ __forceinline__ float clipf(float v, float lo, float hi) { return v <= lo ? lo : (v >= hi ? hi : v); }
 
 __device__ __forceinline__ void store_sample(void* base, int bits, size_t idx, float v) {
@@ -256,124 +262,29 @@ __device__ __forceinline__ float rcp_host(const float* __restrict__ tbl, int shi
 }
 
 // ---------------------------------------------------------------------------
-// ordered gather aggregation shared by the spatial and temporal kernels
+// ordered aggregation into an LDS tile, shared by the spatial and temporal
+// kernels
 // ---------------------------------------------------------------------------
+//
+// One wave owns a kAggTH x kAggTW tile of numerator and denominator
+// accumulators in LDS.  It walks the reference blocks that can reach the tile
+// in scan order and, for each block, its members in group order; every member
+// overlapping the tile is a record.  A record's B*B contributions are loaded
+// contiguously (lane = element of the block) and added into the tile: within
+// one record every lane owns a distinct pixel, and records are applied one
+// after another, so each pixel receives its terms in (block, member) order.
 
-constexpr int kAggCap = 256;   // records buffered per wave (LDS)
-constexpr int kAggWords = 5;   // per record: position, contribution base, den weight x 3
+constexpr int kAggTH = 16;               // tile rows
+constexpr int kAggTW = 32;               // tile columns
+constexpr int kAggStride = kAggTW + 8;   // LDS row stride (8 rows of a block hit distinct banks)
+constexpr int kAggPlane = kAggTH * kAggStride;
+constexpr int kAggLds = 6 * kAggPlane;   // num, den for three planes
 
-// Adds the buffered records [0, nrec) in order.  Loads of 8 records are
-// issued before any of them is added, so their latencies overlap; the adds
-// themselves keep the record order.
-__device__ __forceinline__ void agg_consume(const int* rec, int nrec, int B, int px, int py, bool valid,
-                                            const bool* proc, const float* __restrict__ c0,
-                                            const float* __restrict__ c1, const float* __restrict__ c2, float* num,
-                                            float* den) {
-    const float* cp[3] = {c0, c1, c2};
-    const int BB = B * B;
-    for (int r0 = 0; r0 < nrec; r0 += 8) {
-        float v[3][8], w[3][8];
-        bool cov[8];
-#pragma unroll
-        for (int rr = 0; rr < 8; ++rr) {
-            const int r = r0 + rr;
-            cov[rr] = false;
-            if (r < nrec) {
-                const int* e = rec + r * kAggWords;
-                const int q = e[0];
-                const int dy = py - (q >> 16), dx = px - (q & 0xffff);
-                cov[rr] = valid && dy >= 0 && dy < B && dx >= 0 && dx < B;
-                const size_t off = (size_t)e[1] * BB + (size_t)(dy * B + dx);
-#pragma unroll
-                for (int pl = 0; pl < 3; ++pl) {
-                    if (!proc[pl]) continue;
-                    v[pl][rr] = cov[rr] ? cp[pl][off] : 0.0f;
-                    w[pl][rr] = __int_as_float(e[2 + pl]);
-                }
-            }
-        }
-#pragma unroll
-        for (int rr = 0; rr < 8; ++rr) {
-            if (r0 + rr < nrec && cov[rr]) {
-#pragma unroll
-                for (int pl = 0; pl < 3; ++pl) {
-                    if (!proc[pl]) continue;
-                    num[pl] = num[pl] + v[pl][rr];
-                    den[pl] = den[pl] + w[pl][rr];
-                }
-            }
-        }
-    }
-}
-
-// Walks the reference blocks of the rectangle [a_lo, a_hi] x [c_lo, c_hi] in
-// scan order and accumulates every member (of frame fr, when frame != null)
-// that overlaps the 8x8 tile at (y0, x0), in (block, member) order.
-__device__ inline void agg_tile(const StageParams& p, const int* __restrict__ count, const int* __restrict__ pos,
-                                const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi,
-                                int y0, int x0, int px, int py, bool valid, const bool* proc,
-                                const float* const* contrib, const float* const* denw, int* rec, float* num,
-                                float* den) {
-    const int lane = lane_id();
-    const int B = p.block, G = p.group;
-    const int gp = G <= 1 ? 1 : G <= 2 ? 2 : G <= 4 ? 4 : G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 64;
-    const int per = 64 / gp;  // reference blocks per wave-load
-    const int ncol = c_hi - c_lo + 1;
-    const int chunks_per_row = (ncol + per - 1) / per;
-    const int zchunks = (G + 63) / 64;
-    const int total = (a_hi - a_lo + 1) * chunks_per_row * zchunks;
-    int nrec = 0;
-    for (int it = 0; it < total; ++it) {
-        const int zc = it % zchunks;
-        const int rc = it / zchunks;
-        const int ar = a_lo + rc / chunks_per_row;
-        const int ac = c_lo + (rc % chunks_per_row) * per + lane / gp;
-        const int z = zc * 64 + lane % gp;
-        bool ov = false;
-        int pk = 0, b = 0;
-        float dw[3] = {0.0f, 0.0f, 0.0f};
-        if (ac <= c_hi && z < G) {
-            b = ar * p.nx + ac;
-            const int n = count[b];
-            pk = pos[(size_t)b * G + z];
-            const int fz = frame ? frame[(size_t)b * G + z] : fr;
-#pragma unroll
-            for (int pl = 0; pl < 3; ++pl)
-                if (proc[pl]) dw[pl] = denw[pl][b];
-            const int qy = pk >> 16, qx = pk & 0xffff;
-            ov = z < n && fz == fr && qy <= y0 + 7 && qy + B > y0 && qx <= x0 + 7 && qx + B > x0;
-        }
-        const uint64_t m = __ballot(ov);
-        const int cnt = __popcll(m);
-        if (cnt == 0) continue;
-        if (nrec + cnt > kAggCap) {
-            wave_sync_lds();
-            agg_consume(rec, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], num, den);
-            wave_sync_lds();
-            nrec = 0;
-        }
-        const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        if (ov) {
-            int* e = rec + at * kAggWords;
-            e[0] = pk;
-            e[1] = b * G + z;
-            e[2] = __float_as_int(dw[0]);
-            e[3] = __float_as_int(dw[1]);
-            e[4] = __float_as_int(dw[2]);
-        }
-        nrec += cnt;
-    }
-    wave_sync_lds();
-    agg_consume(rec, nrec, B, px, py, valid, proc, contrib[0], contrib[1], contrib[2], num, den);
-}
-
-// Conservative rectangle of reference blocks whose members can reach a tile
-// when they lie at most `reach` from their reference block.
-__device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0, int reach, int& a_lo, int& a_hi,
-                                           int& c_lo, int& c_hi) {
+__device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0, int th, int tw, int reach,
+                                           int& a_lo, int& a_hi, int& c_lo, int& c_hi) {
     const int B = p.block, s = p.block_step;
-    const int lo_y = y0 - B + 1 - reach, hi_y = y0 + 7 + reach;
-    const int lo_x = x0 - B + 1 - reach, hi_x = x0 + 7 + reach;
+    const int lo_y = y0 - B + 1 - reach, hi_y = y0 + th - 1 + reach;
+    const int lo_x = x0 - B + 1 - reach, hi_x = x0 + tw - 1 + reach;
     a_lo = lo_y > 0 ? lo_y / s - 1 : 0;
     a_hi = hi_y / s + 1;
     c_lo = lo_x > 0 ? lo_x / s - 1 : 0;
@@ -382,6 +293,168 @@ __device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0,
     c_lo = c_lo < 0 ? 0 : c_lo;
     a_hi = a_hi > p.ny - 1 ? p.ny - 1 : a_hi;
     c_hi = c_hi > p.nx - 1 ? p.nx - 1 : c_hi;
+}
+
+// Accumulates, into the LDS tile acc at (y0, x0), every member (of frame fr
+// when frame != null) of the reference blocks [a_lo, a_hi] x [c_lo, c_hi].
+// NP processed planes with contributions c[k] and weights d[k];
+// acc layout: [k][num | den][kAggTH][kAggStride].
+//
+// Loads are issued unconditionally from addresses that are always inside the
+// record (lanes past B*B re-read element 0) and the results of uncovered
+// lanes are simply not added, so a batch of records keeps all its loads in
+// flight at once.
+template <int NP>
+__device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __restrict__ count, const int* __restrict__ pos,
+                                    const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi,
+                                    int y0, int x0, const float* const* c, const float* const* d, float* acc) {
+    const int lane = lane_id();
+    for (int i = lane; i < NP * 2 * kAggPlane; i += 64) acc[i] = 0.0f;
+    wave_sync_lds();
+    const int B = p.block, G = p.group, BB = B * B;
+    const int gp = G <= 1 ? 1 : G <= 2 ? 2 : G <= 4 ? 4 : G <= 8 ? 8 : G <= 16 ? 16 : G <= 32 ? 32 : 64;
+    const int per = 64 / gp;  // reference blocks per walk step
+    const int ncol = c_hi - c_lo + 1;
+    const int chunks_per_row = (ncol + per - 1) / per;
+    const int zchunks = (G + 63) / 64;
+    const int total = (a_hi - a_lo + 1) * chunks_per_row * zchunks;
+    const int y1 = y0 + kAggTH - 1, x1 = x0 + kAggTW - 1;
+    const int lc = lane < BB ? lane : 0;  // element this lane carries (B*B <= 64 path)
+    const int dy = lc / B, dx = lc - (lc / B) * B;
+    const bool small = BB <= 64;
+
+    // walk step it: lane -> (reference block b, member z); loads prefetched
+    // one step ahead
+    auto step_ids = [&](int it, int& b, int& z, bool& live) {
+        const int zc = it % zchunks;
+        const int rc = it / zchunks;
+        const int ar = a_lo + rc / chunks_per_row;
+        const int ac = c_lo + (rc % chunks_per_row) * per + lane / gp;
+        z = zc * 64 + lane % gp;
+        live = ac <= c_hi && z < G;
+        b = ar * p.nx + (ac <= c_hi ? ac : c_hi);
+        z = z < G ? z : G - 1;
+    };
+    int nb_n = 0, pk_n = 0, fz_n = fr;
+    if (total > 0) {
+        int b, z;
+        bool live;
+        step_ids(0, b, z, live);
+        nb_n = count[b];
+        pk_n = pos[b * G + z];
+        if (frame) fz_n = frame[b * G + z];
+    }
+    for (int it = 0; it < total; ++it) {
+        int b, z;
+        bool live;
+        step_ids(it, b, z, live);
+        const int n = nb_n, pk = pk_n, fz = fz_n;
+        if (it + 1 < total) {
+            int b2, z2;
+            bool l2;
+            step_ids(it + 1, b2, z2, l2);
+            nb_n = count[b2];
+            pk_n = pos[b2 * G + z2];
+            if (frame) fz_n = frame[b2 * G + z2];
+        }
+        const int e = b * G + z;
+        const int qy = pk >> 16, qx = pk & 0xffff;
+        const bool ov = live && z < n && fz == fr && qy <= y1 && qy + B > y0 && qx <= x1 && qx + B > x0;
+        uint64_t m = __ballot(ov);
+        if (small) {
+            while (m) {
+                // up to 8 records: all loads first, then the adds in order
+                int rpk[8], re[8], rb[8];
+                int cnt = 0;
+                int l = __builtin_ctzll(m);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (m) {
+                        l = __builtin_ctzll(m);
+                        m &= m - 1;
+                        cnt = j + 1;
+                    }
+                    rpk[j] = __builtin_amdgcn_readlane(pk, l);
+                    re[j] = __builtin_amdgcn_readlane(e, l);
+                    rb[j] = __builtin_amdgcn_readlane(b, l);
+                }
+                float v[NP][8], w[NP][8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const size_t off = (size_t)re[j] * BB + lc;
+#pragma unroll
+                    for (int k = 0; k < NP; ++k) {
+                        v[k][j] = ld_global(c[k] + off);
+                        w[k][j] = ld_global(d[k] + rb[j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    if (j < cnt) {
+                        const int ty = (rpk[j] >> 16) + dy - y0, tx = (rpk[j] & 0xffff) + dx - x0;
+                        if (lane < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
+                            float* an = acc + ty * kAggStride + tx;
+#pragma unroll
+                            for (int k = 0; k < NP; ++k) {
+                                an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + v[k][j];
+                                an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + w[k][j];
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
+            // B*B > 64: one record at a time, 64 elements per pass
+            while (m) {
+                const int l = __builtin_ctzll(m);
+                m &= m - 1;
+                const int rpk = __builtin_amdgcn_readlane(pk, l);
+                const int re = __builtin_amdgcn_readlane(e, l);
+                const int rb = __builtin_amdgcn_readlane(b, l);
+                const int qy2 = rpk >> 16, qx2 = rpk & 0xffff;
+                for (int c0 = 0; c0 < BB; c0 += 64) {
+                    const int el = c0 + lane;
+                    const int ey = el / B, ex = el - (el / B) * B;
+                    const int ty = qy2 + ey - y0, tx = qx2 + ex - x0;
+                    if (el < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
+                        const size_t off = (size_t)re * BB + el;
+                        float* an = acc + ty * kAggStride + tx;
+#pragma unroll
+                        for (int k = 0; k < NP; ++k) {
+                            an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + ld_global(c[k] + off);
+                            an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + ld_global(d[k] + rb);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    wave_sync_lds();
+}
+
+// Runs agg_tile_lds for the processed planes (k-th processed plane -> its
+// contribution and weight arrays, selected without indexing so the pointers
+// stay in registers); returns their number.
+__device__ __forceinline__ int agg_tile_any(const StageParams& p, const int* process, const int* __restrict__ count,
+                                            const int* __restrict__ pos, const int* __restrict__ frame, int fr,
+                                            int a_lo, int a_hi, int c_lo, int c_hi, int y0, int x0,
+                                            const float* const* contrib, const float* const* denw, float* acc) {
+    const bool p0 = process[0] != 0, p1 = process[1] != 0, p2 = process[2] != 0;
+    const int np = (int)p0 + (int)p1 + (int)p2;
+    const int i0 = p0 ? 0 : p1 ? 1 : 2;
+    const int i1 = (p0 && p1) ? 1 : 2;
+    const float* c[3];
+    const float* d[3];
+    c[0] = i0 == 0 ? contrib[0] : i0 == 1 ? contrib[1] : contrib[2];
+    d[0] = i0 == 0 ? denw[0] : i0 == 1 ? denw[1] : denw[2];
+    c[1] = i1 == 1 ? contrib[1] : contrib[2];
+    d[1] = i1 == 1 ? denw[1] : denw[2];
+    c[2] = contrib[2];
+    d[2] = denw[2];
+    if (np == 3) agg_tile_lds<3>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
+    else if (np == 2) agg_tile_lds<2>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
+    else if (np == 1) agg_tile_lds<1>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
+    return np;
 }
 
 }  // namespace bm3d

@@ -921,9 +921,10 @@ hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order,
 // For each output pixel the reference adds float(c * numW) into num and
 // denW into den in (reference-block scan order, group member order)
 // (BlockGroup::AddTo / CountTo, include/Block.h:1239-1259, 1326-1345).  One
-// wave owns an 8x8 tile and walks the reference blocks that can reach it in
-// scan order, so every pixel sees its contributions in exactly that order
-// without atomics; then dst = num / den (BM3D_Base.cpp:455-469).
+// wave owns a 32x16 tile of accumulators in LDS and walks the reference blocks
+// that can reach it in scan order, so every pixel sees its contributions in
+// exactly that order without atomics; then dst = num / den
+// (BM3D_Base.cpp:455-469).
 
 // Writes the filtered float triple/single to the destination format.
 __device__ void write_output(const OutputSpec& o, int x, int y, const float* v, const bool* proc) {
@@ -954,30 +955,41 @@ __device__ void write_output(const OutputSpec& o, int x, int y, const float* v, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_aggregate(AggArgs a) {
-    __shared__ int srec[4][kAggWords * kAggCap];
+// Tiles are dealt to the eight XCDs in contiguous runs (workgroup i runs on
+// XCD i % 8), so neighbouring tiles, which read overlapping contributions,
+// share an L2.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per_xcd = (ntiles + 7) >> 3;
+    return (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+}
+
+__global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
+    __shared__ float acc[kAggLds];
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
-    const int tile = blockIdx.x * 4 + wave;
-    const int tiles_x = (a.p.width + 7) >> 3;
-    const int tiles_y = (a.p.height + 7) >> 3;
+    const int tiles_x = (a.p.width + kAggTW - 1) / kAggTW;
+    const int tiles_y = (a.p.height + kAggTH - 1) / kAggTH;
+    const int tile = xcd_tile(tiles_x * tiles_y);
     if (tile >= tiles_x * tiles_y) return;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
-    const bool valid = px < a.p.width && py < a.p.height;
-    const int y0 = ty * 8, x0 = tx * 8;
+    const int y0 = ty * kAggTH, x0 = tx * kAggTW;
     int a_lo, a_hi, c_lo, c_hi;
-    reach_rect(a.p, y0, x0, a.reach, a_lo, a_hi, c_lo, c_hi);
+    reach_rect(a.p, y0, x0, kAggTH, kAggTW, a.reach, a_lo, a_hi, c_lo, c_hi);
     bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
-    float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
-    int* rq = srec[wave];
-    agg_tile(a.p, a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
-             a.denw, rq, num, den);
-    if (!valid) return;
-    float v[3];
+    agg_tile_any(a.p, a.process, a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib, a.denw, acc);
+    for (int i = lane; i < kAggTH * kAggTW; i += 64) {
+        const int ly = i / kAggTW, lx = i - ly * kAggTW;
+        const int py = y0 + ly, px = x0 + lx;
+        if (py >= a.p.height || px >= a.p.width) continue;
+        float v[3];
+        int k = 0;
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) v[pl] = proc[pl] ? num[pl] / den[pl] : 0.0f;
-    write_output(a.out, px, py, v, proc);
+        for (int pl = 0; pl < 3; ++pl) {
+            const float* an = acc + k * 2 * kAggPlane + ly * kAggStride + lx;
+            v[pl] = proc[pl] ? an[0] / an[kAggPlane] : 0.0f;
+            k += proc[pl] ? 1 : 0;
+        }
+        write_output(a.out, px, py, v, proc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1163,8 +1175,8 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_aggregate(const AggArgs& a, hipStream_t st) {
-    int tiles = ((a.p.width + 7) / 8) * ((a.p.height + 7) / 8);
-    hipLaunchKernelGGL(k_aggregate, dim3((tiles + 3) / 4), dim3(256), 0, st, a);
+    const int tiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
+    hipLaunchKernelGGL(k_aggregate, dim3(8 * ((tiles + 7) / 8)), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

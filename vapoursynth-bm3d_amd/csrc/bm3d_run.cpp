@@ -403,6 +403,8 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
         fa.src[p] = dp + p;
         fa.ref[p] = dp + 3 + p;
         if (!proc[p]) continue;
+        if ((size_t)nb * G >= ((size_t)1 << 31))
+            return set_error(BM3D_E_UNSUPPORTED, "bm3d: reference blocks x group_size must stay below 2^31 per frame");
         if ((rc = f->contrib[p].ensure(sizeof(float) * (size_t)nb * G * B * B))) return rc;
         if ((rc = f->denw[p].ensure(sizeof(float) * nb))) return rc;
         fa.contrib[p] = f->contrib[p].as<float>();
@@ -638,6 +640,8 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
         fa.src[p] = dp + p * T;
         fa.ref[p] = dp + 3 * T + p * T;
         if (!proc[p]) continue;
+        if ((size_t)nb * G >= ((size_t)1 << 31))
+            return set_error(BM3D_E_UNSUPPORTED, "bm3d: reference blocks x group_size must stay below 2^31 per frame");
         if ((rc = f->contrib[p].ensure(sizeof(float) * (size_t)nb * G * B * B))) return rc;
         if ((rc = f->denw[p].ensure(sizeof(float) * nb))) return rc;
         fa.contrib[p] = f->contrib[p].as<float>();

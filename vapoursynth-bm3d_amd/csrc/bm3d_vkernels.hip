@@ -259,41 +259,43 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 // VBM3D_Process_Base::Kernel writes each member into the numerator and
 // denominator planes of its own frame (BlockGroup Pos3 AddTo / CountTo,
 // include/Block.h:1283-1303, 1368-1387) in reference-block scan order, then
-// member order.  One wave per (8x8 tile, frame): it walks the reference blocks
+// member order.  One wave per (tile, frame): it walks the reference blocks
 // that can reach the tile in that frame and accumulates the members of that
 // frame in exactly that order.  Output slot 2*(radius + f - cur) (num) and
 // +1 (den) of the output stack (VBM3D_Base.h:284-292).
 
-__global__ __launch_bounds__(256) void k_vstack(VStackArgs a) {
-    __shared__ int srec[4][kAggWords * kAggCap];
+__global__ __launch_bounds__(64) void k_vstack(VStackArgs a) {
+    __shared__ float acc[kAggLds];
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
-    const int tiles_x = (a.p.width + 7) >> 3, tiles_y = (a.p.height + 7) >> 3;
-    const int wid = blockIdx.x * 4 + wave;
-    if (wid >= tiles_x * tiles_y * a.nframes) return;
-    const int fr = wid / (tiles_x * tiles_y);
-    const int tile = wid - fr * tiles_x * tiles_y;
+    const int tiles_x = (a.p.width + kAggTW - 1) / kAggTW, tiles_y = (a.p.height + kAggTH - 1) / kAggTH;
+    const int ntiles = tiles_x * tiles_y;
+    const int fr = blockIdx.y;
+    const int per_xcd = (ntiles + 7) >> 3;
+    const int tile = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (tile >= ntiles) return;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    const int px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
-    const bool valid = px < a.p.width && py < a.p.height;
-    const int y0 = ty * 8, x0 = tx * 8;
+    const int y0 = ty * kAggTH, x0 = tx * kAggTW;
     const int dist = fr > a.cur ? fr - a.cur : a.cur - fr;
     const int reach = a.p.range + dist * (a.p.ps_range / a.p.ps_step * a.p.ps_step);
     int a_lo, a_hi, c_lo, c_hi;
-    reach_rect(a.p, y0, x0, reach, a_lo, a_hi, c_lo, c_hi);
-    bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
-    float num[3] = {0.0f, 0.0f, 0.0f}, den[3] = {0.0f, 0.0f, 0.0f};
-    int* rq = srec[wave];
-    agg_tile(a.p, a.count, a.pos, a.frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, px, py, valid, proc, a.contrib,
-             a.denw, rq, num, den);
-    if (!valid) return;
+    reach_rect(a.p, y0, x0, kAggTH, kAggTW, reach, a_lo, a_hi, c_lo, c_hi);
+    agg_tile_any(a.p, a.process, a.count, a.pos, a.frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib, a.denw,
+                 acc);
     const int slot = 2 * (a.p.radius + fr - a.cur);
+    for (int i = lane; i < kAggTH * kAggTW; i += 64) {
+        const int ly = i / kAggTW, lx = i - ly * kAggTW;
+        const int py = y0 + ly, px = x0 + lx;
+        if (py >= a.p.height || px >= a.p.width) continue;
+        int k = 0;
 #pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {
-        if (!proc[pl]) continue;
-        float* basep = a.dst[pl];
-        basep[((size_t)slot * a.p.height + py) * a.dst_pitch[pl] + px] = num[pl];
-        basep[((size_t)(slot + 1) * a.p.height + py) * a.dst_pitch[pl] + px] = den[pl];
+        for (int pl = 0; pl < 3; ++pl) {
+            if (!a.process[pl]) continue;
+            const float* an = acc + k * 2 * kAggPlane + ly * kAggStride + lx;
+            ++k;
+            float* basep = a.dst[pl];
+            basep[((size_t)slot * a.p.height + py) * a.dst_pitch[pl] + px] = an[0];
+            basep[((size_t)(slot + 1) * a.p.height + py) * a.dst_pitch[pl] + px] = an[kAggPlane];
+        }
     }
 }
 
@@ -345,8 +347,8 @@ hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_vstack(const VStackArgs& a, hipStream_t st) {
-    int tiles = ((a.p.width + 7) / 8) * ((a.p.height + 7) / 8) * a.nframes;
-    hipLaunchKernelGGL(k_vstack, dim3((tiles + 3) / 4), dim3(256), 0, st, a);
+    const int tiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
+    hipLaunchKernelGGL(k_vstack, dim3(8 * ((tiles + 7) / 8), a.nframes), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
