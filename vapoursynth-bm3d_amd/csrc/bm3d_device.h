@@ -232,6 +232,13 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// arr[i] for a three-entry array of a kernel-argument struct, by selects: a
+// dynamically indexed kernel-argument array would be copied to scratch.
+template <typename T>
+__device__ __forceinline__ T pick3(const T (&arr)[3], int i) {
+    return i == 0 ? arr[0] : (i == 1 ? arr[1] : arr[2]);
+}
+
 // Load from global memory (the pointer may have lost its address space
 // through a select; a flat load would also count against the LDS counter).
 __device__ __forceinline__ float ld_global(const float* p) {

@@ -168,7 +168,9 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __rest
     const int l = search_lo(i, p.range, 1), r = search_hi(i, p.width - B, p.range, 1);
     const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
     const int ncols = r - l + 1, nrows = bt - t + 1;
-    const int WW = ncols + 7 + 8;  // window row incl. slack for the last strip
+    // window row incl. slack for the last strip; odd, so that lanes on
+    // consecutive rows read distinct LDS banks
+    const int WW = (ncols + 7 + 8) | 1;
     const int WH = nrows + 7;
     for (int e = lane; e < WH * WW; e += 64) {
         int wy = e / WW, wx = e - wy * WW;
@@ -189,10 +191,12 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __rest
     int naccepted = 0;
 #pragma unroll
     for (int tk = 0; tk < KMAX; ++tk) {
+        // lanes take consecutive rows of one strip (row-fastest order)
         const int task = lane + 64 * tk;
         const bool active = task < tasks;
-        const int cy = active ? task / nstr : 0;
-        const int sx = active ? (task - cy * nstr) * 8 : 0;
+        const int strip = active ? task / nrows : 0;
+        const int cy = active ? task - strip * nrows : 0;
+        const int sx = strip * 8;
         float s[8][4];
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -232,8 +236,8 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __rest
     // candidate index of register slot (tk, k): cy * ncols + cx (row-major lattice order)
     auto cand_idx = [&](int tk, int k) {
         int task = lane + 64 * tk;
-        int cy = task / nstr;
-        return cy * ncols + (task - cy * nstr) * 8 + k;
+        int strip = task / nrows;
+        return (task - strip * nrows) * ncols + strip * 8 + k;
     };
     const int kk = naccepted < G - 1 ? naccepted : G - 1;
     const bool cut = naccepted > kk;
@@ -718,12 +722,14 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
     const int y = lane >> 3, x = lane & 7;
     const int off = y * 9 + x;
     for (int pl = 0; pl < 3; ++pl) {
-        if (!a.process[pl]) continue;
+        if (!pick3(a.process, pl)) continue;
+        const float* const* srcp = pick3(a.src, pl);
+        const float* const* refp = pick3(a.ref, pl);
         const PlaneTables& tb = a.tables[pl];
         float X[N];
         float den_w;
         if (!WIENER) {
-            gather8<N>(gs, a.src[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
+            gather8<N>(gs, srcp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
             float thr4[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) thr4[q] = tb.thr[N - 1][q];
@@ -743,12 +749,12 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         } else {
             // one LDS buffer: the reference group's transform goes to registers
             // first, then the source group reuses the buffer
-            gather8<N>(gs, a.ref[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
+            gather8<N>(gs, refp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
             passes_xy<true>(gs, N);
             float R[N];
             zfwdN<N>(gs, off, R);
             wave_sync_lds();
-            gather8<N>(gs, a.src[pl], zyx, a.p.pitch, off, y, x, a.frame == nullptr);
+            gather8<N>(gs, srcp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
             passes_xy<true>(gs, N);
             zfwdN<N>(gs, off, X);
             wave_sync_lds();
@@ -796,10 +802,10 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         wave_sync_lds();
         passes_xy<false>(gs, N);
         const float num_w = (float)((double)den_w / tb.final_amp[N - 1]);
-        float* contrib = a.contrib[pl] + (size_t)g_id * G * 64;
+        float* contrib = pick3(a.contrib, pl) + (size_t)g_id * G * 64;
 #pragma unroll
         for (int z = 0; z < N; ++z) contrib[z * 64 + lane] = gs[z * 72 + off] * num_w;
-        if (lane == 0) a.denw[pl][g_id] = den_w;
+        if (lane == 0) pick3(a.denw, pl)[g_id] = den_w;
         wave_sync_lds();
     }
 }
@@ -1072,7 +1078,7 @@ int match8_kmax(const StageParams& p, int max_cand) {
 
 size_t match8_wave_words(int max_cand, int G) {
     int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
-    return (size_t)(side + 7) * (side + 15) + 4 * (size_t)G;
+    return (size_t)(side + 7) * ((side + 15) | 1) + 4 * (size_t)G;
 }
 
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
