@@ -39,7 +39,7 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
                                                             int* __restrict__ out_pos,
                                                             float* __restrict__ out_key) {
     extern __shared__ uint32_t smem[];
-    const int wave = threadIdx.x >> 6;
+    const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
     const int b = b0 + blockIdx.x * kMatchWaves + wave;
     if (b >= nb) return;
@@ -135,22 +135,35 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
 }
 
 // Fast path for B = 8 and bm_step = 1: the search window is staged in LDS
-// and each lane scores a horizontal strip of 8 neighbouring candidates, so a
-// window row of 15 samples feeds 64 differences (15 LDS reads per 8x8 SSD
-// instead of 64).  The per-candidate arithmetic is exactly ssd_fixed<8>'s.
-// Keys stay in registers; selection is a radix descent on ballot counts.
-template <int KMAX>
-__global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __restrict__ plane, StageParams p, int nb,
+// and each lane scores a horizontal strip of W neighbouring candidates, so a
+// window row of W + 7 samples feeds 8W differences.  Lanes take consecutive
+// rows of a strip and the window row stride is odd, so the lanes of a wave
+// read distinct LDS banks; the reference block is read from LDS by broadcast.
+// The per-candidate arithmetic is exactly ssd_fixed<8>'s.  Keys stay in
+// registers.
+//
+// Selection of the k best (key, candidate index) pairs: the k-th smallest of
+// the per-lane minimum keys, t0, bounds the k-th smallest key (k lane minima
+// are <= t0), so every selected pair has key <= t0.  Those pairs (usually a
+// few dozen) are compacted into LDS and the exact threshold is found by a
+// radix descent over them alone; if they do not fit, the descent runs over all
+// keys in registers.
+constexpr int kSelCap = 256;
+
+template <int KMAX, int W>
+__global__ __launch_bounds__(64 * kMatchWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_match8(const float* __restrict__ plane, StageParams p, int nb,
                                                              int wave_words, int* __restrict__ out_count,
                                                              int* __restrict__ out_pos, float* __restrict__ out_key) {
     extern __shared__ uint32_t smem[];
-    const int wave = threadIdx.x >> 6;
+    const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
     const int b = blockIdx.x * kMatchWaves + wave;
     if (b >= nb) return;
     constexpr int B = 8;
+    constexpr int NK = KMAX * W;
     const int G = p.group;
-    float* win = reinterpret_cast<float*>(smem + (size_t)wave * wave_words);
+    uint32_t* wbase = smem + (size_t)wave * wave_words;
+    float* win = reinterpret_cast<float*>(wbase);
     const int bj = b / p.nx, bi = b - bj * p.nx;
     const int j = grid_pos(bj, p.block_step, p.height - B);
     const int i = grid_pos(bi, p.block_step, p.width - B);
@@ -168,125 +181,217 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match8(const float* __rest
     const int l = search_lo(i, p.range, 1), r = search_hi(i, p.width - B, p.range, 1);
     const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
     const int ncols = r - l + 1, nrows = bt - t + 1;
-    // window row incl. slack for the last strip; odd, so that lanes on
-    // consecutive rows read distinct LDS banks
-    const int WW = (ncols + 7 + 8) | 1;
+    const int nstr = (ncols + W - 1) / W;
+    const int WW = (nstr * W + 7) | 1;  // covers the last strip's W + 7 samples
     const int WH = nrows + 7;
     for (int e = lane; e < WH * WW; e += 64) {
         int wy = e / WW, wx = e - wy * WW;
         int gx = l + wx;
         win[e] = gx < p.width ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
     }
-    uint32_t* sel_key = smem + (size_t)wave * wave_words + WH * WW;
-    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);
-    wave_sync_lds();
-    float rb[64];
+    const int win_words = (WH * WW + 3) & ~3;
+    float* rbs = win + win_words;                                   // [64] reference block
+    uint32_t* s_key = wbase + win_words + 64;                       // [kSelCap]
+    int* s_idx = reinterpret_cast<int*>(s_key + kSelCap);           // [kSelCap]
+    uint32_t* sel_key = reinterpret_cast<uint32_t*>(s_idx + kSelCap);  // [2G]
+    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);         // [2G]
     const int oy = j - t, ox = i - l;
-#pragma unroll
-    for (int e = 0; e < 64; ++e) rb[e] = win[(oy + e / 8) * WW + ox + e % 8];
+    wave_sync_lds();
+    rbs[lane] = win[(oy + (lane >> 3)) * WW + ox + (lane & 7)];
+    wave_sync_lds();
 
-    const int nstr = (ncols + 7) >> 3;
     const int tasks = nrows * nstr;
-    uint32_t keys[KMAX * 8];
+    uint32_t keys[NK];
     int naccepted = 0;
 #pragma unroll
     for (int tk = 0; tk < KMAX; ++tk) {
-        // lanes take consecutive rows of one strip (row-fastest order)
         const int task = lane + 64 * tk;
         const bool active = task < tasks;
         const int strip = active ? task / nrows : 0;
         const int cy = active ? task - strip * nrows : 0;
-        const int sx = strip * 8;
-        float s[8][4];
+        const int sx = strip * W;
+        float s[W][4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
+        for (int k = 0; k < W; ++k)
 #pragma unroll
             for (int c = 0; c < 4; ++c) s[k][c] = 0.0f;
-#pragma unroll
+        // rows one at a time: each row's loads stay with its arithmetic
+#pragma unroll 1
         for (int rr = 0; rr < 8; ++rr) {
+            const float4 ra = *reinterpret_cast<const float4*>(rbs + rr * 8);
+            const float4 rc = *reinterpret_cast<const float4*>(rbs + rr * 8 + 4);
+            const float rv[8] = {ra.x, ra.y, ra.z, ra.w, rc.x, rc.y, rc.z, rc.w};
             const float* wr = win + (cy + rr) * WW + sx;
-            float row[15];
+            float row[W + 7];
 #pragma unroll
-            for (int q = 0; q < 15; ++q) row[q] = wr[q];
+            for (int q = 0; q < W + 7; ++q) row[q] = wr[q];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < W; ++k) {
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    float d = rb[rr * 8 + c] - row[k + c];
+                    float d = rv[c] - row[k + c];
                     s[k][c] = s[k][c] + d * d;
                 }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    float d = rb[rr * 8 + 4 + c] - row[k + 4 + c];
+                    float d = rv[4 + c] - row[k + 4 + c];
                     s[k][c] = s[k][c] + d * d;
                 }
             }
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < W; ++k) {
             float dist = 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
             const int cx = sx + k;
             const bool valid = active && cx < ncols && !(cy == oy && cx == ox);
             uint32_t kv = 0xFFFFFFFFu;
             if (valid && dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
-            keys[tk * 8 + k] = kv;
+            keys[tk * W + k] = kv;
             naccepted += __popcll(__ballot(kv != 0xFFFFFFFFu));
         }
     }
-    // candidate index of register slot (tk, k): cy * ncols + cx (row-major lattice order)
-    auto cand_idx = [&](int tk, int k) {
-        int task = lane + 64 * tk;
-        int strip = task / nrows;
-        return (task - strip * nrows) * ncols + strip * 8 + k;
+    // candidate index of register slot q = tk * W + k: cy * ncols + cx (row-major lattice order)
+    auto cand_idx = [&](int q) {
+        const int task = lane + 64 * (q / W);
+        const int strip = task / nrows;
+        return (task - strip * nrows) * ncols + strip * W + q % W;
     };
     const int kk = naccepted < G - 1 ? naccepted : G - 1;
     const bool cut = naccepted > kk;
-    uint32_t thr = 0xFFFFFFFEu;
-    int ithr = 0x7FFFFFFF;  // among keys == thr, admit candidate indices < ithr
-    if (cut) {
-        uint32_t tv = 0;
+    int nsel = 0;  // entries written to sel_key/sel_idx[0..kk)
+    if (!cut) {
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const bool sel = keys[q] != 0xFFFFFFFFu;
+            const uint64_t m = __ballot(sel);
+            const int at = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            if (sel) {
+                sel_key[at] = keys[q];
+                sel_idx[at] = cand_idx(q);
+            }
+            nsel += __popcll(m);
+        }
+    } else {
+        // bound: t0 = kk-th smallest lane minimum
+        uint32_t lm = 0xFFFFFFFFu;
+#pragma unroll
+        for (int q = 0; q < NK; ++q) lm = keys[q] < lm ? keys[q] : lm;
+        uint32_t t0 = 0;
         for (int bit = 31; bit >= 0; --bit) {
-            const uint32_t trial = tv | (1u << bit);
-            int cnt = 0;
-#pragma unroll
-            for (int q = 0; q < KMAX * 8; ++q) cnt += __popcll(__ballot(keys[q] < trial));
-            if (cnt < kk) tv = trial;
+            const uint32_t trial = t0 | (1u << bit);
+            if (__popcll(__ballot(lm < trial)) < kk) t0 = trial;
         }
-        int less = 0, eq = 0;
+        int M = 0;
+        if (t0 != 0xFFFFFFFFu) {
 #pragma unroll
-        for (int q = 0; q < KMAX * 8; ++q) {
-            less += __popcll(__ballot(keys[q] < tv));
-            eq += __popcll(__ballot(keys[q] == tv));
+            for (int q = 0; q < NK; ++q) M += __popcll(__ballot(keys[q] <= t0));
         }
-        thr = tv;
-        const int need = kk - less;
-        if (eq > need) {
-            // equal keys at the cut: the earliest candidates win
-            int it = 0;
-            for (int bit = 20; bit >= 0; --bit) {
-                const int trial = it | (1 << bit);
+        if (t0 != 0xFFFFFFFFu && M <= kSelCap) {
+            int base = 0;
+#pragma unroll
+            for (int q = 0; q < NK; ++q) {
+                const bool in = keys[q] <= t0;
+                const uint64_t m = __ballot(in);
+                const int at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (in) {
+                    s_key[at] = keys[q];
+                    s_idx[at] = cand_idx(q);
+                }
+                base += __popcll(m);
+            }
+            wave_sync_lds();
+            constexpr int SR = kSelCap / 64;
+            uint32_t ck[SR];
+            int ci[SR];
+#pragma unroll
+            for (int u = 0; u < SR; ++u) {
+                const int e = lane + 64 * u;
+                ck[u] = e < M ? s_key[e] : 0xFFFFFFFFu;
+                ci[u] = e < M ? s_idx[e] : 0x7FFFFFFF;
+            }
+            // exact k-th smallest key among the compacted pairs
+            uint32_t tv = 0;
+            for (int bit = 31; bit >= 0; --bit) {
+                const uint32_t trial = tv | (1u << bit);
                 int cnt = 0;
 #pragma unroll
-                for (int q = 0; q < KMAX * 8; ++q)
-                    cnt += __popcll(__ballot(keys[q] == tv && cand_idx(q >> 3, q & 7) < trial));
-                if (cnt < need) it = trial;
+                for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] < trial));
+                if (cnt < kk) tv = trial;
             }
-            ithr = it + 1;  // indices <= it
-        }
-    }
-    int base = 0;
+            int less = 0, eq = 0;
 #pragma unroll
-    for (int q = 0; q < KMAX * 8; ++q) {
-        const uint32_t kv = keys[q];
-        const int ci = cand_idx(q >> 3, q & 7);
-        const bool sel = cut ? (kv < thr || (kv == thr && ci < ithr)) : (kv != 0xFFFFFFFFu);
-        const uint64_t m = __ballot(sel);
-        const int at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-        if (sel) {
-            sel_key[at] = kv;
-            sel_idx[at] = ci;
+            for (int u = 0; u < SR; ++u) {
+                less += __popcll(__ballot(ck[u] < tv));
+                eq += __popcll(__ballot(ck[u] == tv));
+            }
+            const int need = kk - less;
+            int ithr = 0x7FFFFFFF;
+            if (eq > need) {
+                // equal keys at the cut: the earliest candidates win
+                int it = 0;
+                for (int bit = 20; bit >= 0; --bit) {
+                    const int trial = it | (1 << bit);
+                    int cnt = 0;
+#pragma unroll
+                    for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] == tv && ci[u] < trial));
+                    if (cnt < need) it = trial;
+                }
+                ithr = it + 1;
+            }
+#pragma unroll
+            for (int u = 0; u < SR; ++u) {
+                const bool sel = ck[u] < tv || (ck[u] == tv && ci[u] < ithr);
+                const uint64_t m = __ballot(sel);
+                const int at = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (sel) {
+                    sel_key[at] = ck[u];
+                    sel_idx[at] = ci[u];
+                }
+                nsel += __popcll(m);
+            }
+        } else {
+            // descent over every key in registers
+            uint32_t tv = 0;
+            for (int bit = 31; bit >= 0; --bit) {
+                const uint32_t trial = tv | (1u << bit);
+                int cnt = 0;
+#pragma unroll
+                for (int q = 0; q < NK; ++q) cnt += __popcll(__ballot(keys[q] < trial));
+                if (cnt < kk) tv = trial;
+            }
+            int less = 0, eq = 0;
+#pragma unroll
+            for (int q = 0; q < NK; ++q) {
+                less += __popcll(__ballot(keys[q] < tv));
+                eq += __popcll(__ballot(keys[q] == tv));
+            }
+            const int need = kk - less;
+            int ithr = 0x7FFFFFFF;
+            if (eq > need) {
+                int it = 0;
+                for (int bit = 20; bit >= 0; --bit) {
+                    const int trial = it | (1 << bit);
+                    int cnt = 0;
+#pragma unroll
+                    for (int q = 0; q < NK; ++q) cnt += __popcll(__ballot(keys[q] == tv && cand_idx(q) < trial));
+                    if (cnt < need) it = trial;
+                }
+                ithr = it + 1;
+            }
+#pragma unroll
+            for (int q = 0; q < NK; ++q) {
+                const uint32_t kv = keys[q];
+                const int ci = cand_idx(q);
+                const bool sel = kv < tv || (kv == tv && ci < ithr);
+                const uint64_t m = __ballot(sel);
+                const int at = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (sel) {
+                    sel_key[at] = kv;
+                    sel_idx[at] = ci;
+                }
+                nsel += __popcll(m);
+            }
         }
-        base += __popcll(m);
     }
     wave_sync_lds();
     rank_sort_selected(sel_key, sel_idx, kk);
@@ -1069,16 +1174,29 @@ size_t match_lds_bytes(int max_cand, int B, int G) {
     return sizeof(uint32_t) * (size_t)kMatchWaves * match_wave_words(max_cand, B, G);
 }
 
-int match8_kmax(const StageParams& p, int max_cand) {
-    // tasks = rows x strips of 8 for the largest (interior) window
-    int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
-    int tasks = side * ((side + 7) / 8);
-    return (tasks + 63) / 64;
-}
+// Strip width and task passes of the B = 8 fast path for the largest
+// (interior) window: the width with the fewest scored slots (ties: 8).
+struct Match8Shape {
+    int w, kmax;
+    size_t words;
+};
 
-size_t match8_wave_words(int max_cand, int G) {
-    int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
-    return (size_t)(side + 7) * ((side + 15) | 1) + 4 * (size_t)G;
+Match8Shape match8_shape(int max_cand, int G) {
+    const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
+    Match8Shape best{0, 0, 0};
+    long best_slots = -1;
+    for (int w : {8, 11}) {
+        const int nstr = (side + w - 1) / w;
+        const int kmax = (side * nstr + 63) / 64;
+        const long slots = (long)kmax * 64 * w;
+        if (best_slots < 0 || slots < best_slots) {
+            best_slots = slots;
+            const size_t ww = (size_t)((nstr * w + 7) | 1);
+            const size_t win = ((size_t)(side + 7) * ww + 3) & ~(size_t)3;
+            best = {w, kmax, win + 64 + 2 * (size_t)kSelCap + 4 * (size_t)G};
+        }
+    }
+    return best;
 }
 
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
@@ -1086,23 +1204,24 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
     int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
     if (blocks == 0) return hipSuccess;
     if (p.block == 8 && p.step == 1) {
-        const int km = match8_kmax(p, max_cand);
-        const size_t ww = match8_wave_words(max_cand, p.group);
-        const size_t lds = ww * 4 * kMatchWaves;
-        if (km <= 8 && lds <= 65536) {
-#define BM3D_M8(K)                                                                                      \
-    case K:                                                                                            \
-        hipLaunchKernelGGL(k_match8<K>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, nb, (int)ww, \
-                           count, pos, key);                                                           \
-        return hipGetLastError();
-            switch (km <= 1 ? 1 : km <= 2 ? 2 : km <= 3 ? 3 : km <= 4 ? 4 : km <= 6 ? 6 : 8) {
-                BM3D_M8(1)
-                BM3D_M8(2)
-                BM3D_M8(3)
-                BM3D_M8(4)
-                BM3D_M8(6)
-                BM3D_M8(8)
-            }
+        const Match8Shape sh = match8_shape(max_cand, p.group);
+        const size_t lds = sh.words * 4 * kMatchWaves;
+        const int km = sh.kmax <= 1 ? 1 : sh.kmax <= 2 ? 2 : sh.kmax <= 3 ? 3 : sh.kmax <= 4 ? 4 : 0;
+        if (km != 0 && lds <= 65536) {
+#define BM3D_M8(K, WD)                                                                                        \
+    if (km == K && sh.w == WD) {                                                                              \
+        hipLaunchKernelGGL((k_match8<K, WD>), dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, nb,    \
+                           (int)sh.words, count, pos, key);                                                   \
+        return hipGetLastError();                                                                             \
+    }
+            BM3D_M8(1, 8)
+            BM3D_M8(2, 8)
+            BM3D_M8(3, 8)
+            BM3D_M8(4, 8)
+            BM3D_M8(1, 11)
+            BM3D_M8(2, 11)
+            BM3D_M8(3, 11)
+            BM3D_M8(4, 11)
 #undef BM3D_M8
         }
     }

@@ -92,7 +92,7 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
 template <int BT>
 __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     extern __shared__ uint32_t vsm[];
-    const int wave = threadIdx.x >> 6;
+    const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
     const int b = blockIdx.x * kVMatchWaves + wave;
     const StageParams& p = a.p;
