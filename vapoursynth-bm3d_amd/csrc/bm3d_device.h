@@ -302,19 +302,104 @@ __device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0,
     c_hi = c_hi > p.nx - 1 ? p.nx - 1 : c_hi;
 }
 
+constexpr int kAggRecCap = 512;    // records buffered per wave between walk and consume
+constexpr int kAggRecWords = 3;    // position, member index b*G + z, reference block b
+constexpr int kAggBatch = 8;       // records per load batch of the consume phase
+
+// One batch of kAggBatch records of the consume phase: lane j (< kAggBatch)
+// holds record j's position and weights, every lane its element of each
+// record's contribution block.
+template <int NP>
+struct AggBatch {
+    int pk;                    // lane j: packed position of record j
+    float w[NP];               // lane j: weight of record j
+    float v[NP][kAggBatch];    // element lc of record j's block
+};
+
+// Issues the loads of records [r0, r0 + kAggBatch) (records past nrec repeat
+// the last one and are never applied).  Every lane reads element lc of each
+// block, which is always inside the block.
+template <int NP>
+__device__ __forceinline__ void agg_load_batch(const int* rec, int r0, int nrec, int BB, int lc,
+                                               const float* const* c, const float* const* d, AggBatch<NP>& t) {
+    const int lane = lane_id();
+    const int rj = r0 + (lane < kAggBatch ? lane : 0);
+    const int idx = (rj < nrec ? rj : nrec - 1) * kAggRecWords;
+    t.pk = rec[idx];
+    const int re_l = rec[idx + 1];
+    const int rb_l = rec[idx + 2];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) t.w[k] = ld_global(d[k] + rb_l);
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) {
+        const size_t off = (size_t)__builtin_amdgcn_readlane(re_l, j) * BB + lc;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) t.v[k][j] = ld_global(c[k] + off);
+    }
+}
+
+// Adds, in order, the records [r0, r0 + kAggBatch) below nrec.
+template <int NP>
+__device__ __forceinline__ void agg_apply_batch(const AggBatch<NP>& t, int r0, int nrec, int BB, int dy, int dx,
+                                                int y0, int x0, float* acc) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < kAggBatch; ++j) {
+        if (r0 + j < nrec) {
+            const int pk = __builtin_amdgcn_readlane(t.pk, j);
+            const int ty = (pk >> 16) + dy - y0, tx = (pk & 0xffff) + dx - x0;
+            if (lane < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
+                float* an = acc + ty * kAggStride + tx;
+#pragma unroll
+                for (int k = 0; k < NP; ++k) {
+                    const float w = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t.w[k]), j));
+                    an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + t.v[k][j];
+                    an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + w;
+                }
+            }
+        }
+    }
+}
+
+// Applies the whole list [0, nrec) in order, with the loads of the next two
+// batches in flight while a batch is being added.
+template <int NP>
+__device__ __forceinline__ void agg_consume_list(const int* rec, int nrec, int B, int lc, int dy, int dx, int y0,
+                                                 int x0, const float* const* c, const float* const* d,
+                                                 float* acc) {
+    if (nrec <= 0) return;
+    const int BB = B * B;
+    constexpr int S = kAggBatch;
+    AggBatch<NP> ta, tb, tc;
+    agg_load_batch<NP>(rec, 0, nrec, BB, lc, c, d, ta);
+    if (S < nrec) agg_load_batch<NP>(rec, S, nrec, BB, lc, c, d, tb);
+    for (int r0 = 0; r0 < nrec; r0 += 3 * S) {
+        if (r0 + 2 * S < nrec) agg_load_batch<NP>(rec, r0 + 2 * S, nrec, BB, lc, c, d, tc);
+        agg_apply_batch<NP>(ta, r0, nrec, BB, dy, dx, y0, x0, acc);
+        if (r0 + S >= nrec) break;
+        if (r0 + 3 * S < nrec) agg_load_batch<NP>(rec, r0 + 3 * S, nrec, BB, lc, c, d, ta);
+        agg_apply_batch<NP>(tb, r0 + S, nrec, BB, dy, dx, y0, x0, acc);
+        if (r0 + 2 * S >= nrec) break;
+        if (r0 + 4 * S < nrec) agg_load_batch<NP>(rec, r0 + 4 * S, nrec, BB, lc, c, d, tb);
+        agg_apply_batch<NP>(tc, r0 + 2 * S, nrec, BB, dy, dx, y0, x0, acc);
+    }
+}
+
 // Accumulates, into the LDS tile acc at (y0, x0), every member (of frame fr
 // when frame != null) of the reference blocks [a_lo, a_hi] x [c_lo, c_hi].
 // NP processed planes with contributions c[k] and weights d[k];
-// acc layout: [k][num | den][kAggTH][kAggStride].
+// acc layout: [k][num | den][kAggTH][kAggStride]; rec: the wave's LDS record
+// list (kAggRecCap x kAggRecWords ints).
 //
-// Loads are issued unconditionally from addresses that are always inside the
-// record (lanes past B*B re-read element 0) and the results of uncovered
-// lanes are simply not added, so a batch of records keeps all its loads in
-// flight at once.
+// Walk: lanes take (reference block, member) pairs in scan order, four steps'
+// loads at a time; members overlapping the tile are appended to the record
+// list.  Consume: the list is applied in order (agg_consume_list).  B*B > 64
+// (blocks larger than a wave) takes a direct path, one record at a time.
 template <int NP>
-__device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __restrict__ count, const int* __restrict__ pos,
-                                    const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo, int c_hi,
-                                    int y0, int x0, const float* const* c, const float* const* d, float* acc) {
+__device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __restrict__ count,
+                                             const int* __restrict__ pos, const int* __restrict__ frame, int fr,
+                                             int a_lo, int a_hi, int c_lo, int c_hi, int y0, int x0,
+                                             const float* const* c, const float* const* d, float* acc, int* rec) {
     const int lane = lane_id();
     for (int i = lane; i < NP * 2 * kAggPlane; i += 64) acc[i] = 0.0f;
     wave_sync_lds();
@@ -330,137 +415,104 @@ __device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __
     const int dy = lc / B, dx = lc - (lc / B) * B;
     const bool small = BB <= 64;
 
-    // walk step it: lane -> (reference block b, member z); loads prefetched
-    // one step ahead
     auto step_ids = [&](int it, int& b, int& z, bool& live) {
         const int zc = it % zchunks;
         const int rc = it / zchunks;
         const int ar = a_lo + rc / chunks_per_row;
         const int ac = c_lo + (rc % chunks_per_row) * per + lane / gp;
         z = zc * 64 + lane % gp;
-        live = ac <= c_hi && z < G;
+        live = it < total && ac <= c_hi && z < G;
         b = ar * p.nx + (ac <= c_hi ? ac : c_hi);
+        b = it < total ? b : a_lo * p.nx + c_lo;
         z = z < G ? z : G - 1;
     };
-    int nb_n = 0, pk_n = 0, fz_n = fr;
-    if (total > 0) {
-        int b, z;
-        bool live;
-        step_ids(0, b, z, live);
-        nb_n = count[b];
-        pk_n = pos[b * G + z];
-        if (frame) fz_n = frame[b * G + z];
-    }
-    for (int it = 0; it < total; ++it) {
-        int b, z;
-        bool live;
-        step_ids(it, b, z, live);
-        const int n = nb_n, pk = pk_n, fz = fz_n;
-        if (it + 1 < total) {
-            int b2, z2;
-            bool l2;
-            step_ids(it + 1, b2, z2, l2);
-            nb_n = count[b2];
-            pk_n = pos[b2 * G + z2];
-            if (frame) fz_n = frame[b2 * G + z2];
+    int nrec = 0;
+    for (int it = 0; it < total; it += 4) {
+        int sb[4], sz[4], sn[4], spk[4], sfz[4];
+        bool slive[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            step_ids(it + u, sb[u], sz[u], slive[u]);
+            sn[u] = count[sb[u]];
+            spk[u] = pos[sb[u] * G + sz[u]];
+            sfz[u] = frame ? frame[sb[u] * G + sz[u]] : fr;
         }
-        const int e = b * G + z;
-        const int qy = pk >> 16, qx = pk & 0xffff;
-        const bool ov = live && z < n && fz == fr && qy <= y1 && qy + B > y0 && qx <= x1 && qx + B > x0;
-        uint64_t m = __ballot(ov);
-        if (small) {
-            while (m) {
-                // up to 8 records: all loads first, then the adds in order
-                int rpk[8], re[8], rb[8];
-                int cnt = 0;
-                int l = __builtin_ctzll(m);
 #pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (m) {
-                        l = __builtin_ctzll(m);
-                        m &= m - 1;
-                        cnt = j + 1;
-                    }
-                    rpk[j] = __builtin_amdgcn_readlane(pk, l);
-                    re[j] = __builtin_amdgcn_readlane(e, l);
-                    rb[j] = __builtin_amdgcn_readlane(b, l);
-                }
-                float v[NP][8], w[NP][8];
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const size_t off = (size_t)re[j] * BB + lc;
-#pragma unroll
-                    for (int k = 0; k < NP; ++k) {
-                        v[k][j] = ld_global(c[k] + off);
-                        w[k][j] = ld_global(d[k] + rb[j]);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    if (j < cnt) {
-                        const int ty = (rpk[j] >> 16) + dy - y0, tx = (rpk[j] & 0xffff) + dx - x0;
-                        if (lane < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
+        for (int u = 0; u < 4; ++u) {
+            const int qy = spk[u] >> 16, qx = spk[u] & 0xffff;
+            const bool ov = slive[u] && sz[u] < sn[u] && sfz[u] == fr && qy <= y1 && qy + B > y0 && qx <= x1 &&
+                            qx + B > x0;
+            uint64_t m = __ballot(ov);
+            if (!small) {
+                // B*B > 64: one record at a time, 64 elements per pass
+                while (m) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1;
+                    const int rpk = __builtin_amdgcn_readlane(spk[u], l);
+                    const int re = __builtin_amdgcn_readlane(sb[u] * G + sz[u], l);
+                    const int rb = __builtin_amdgcn_readlane(sb[u], l);
+                    const int qy2 = rpk >> 16, qx2 = rpk & 0xffff;
+                    for (int c0 = 0; c0 < BB; c0 += 64) {
+                        const int el = c0 + lane;
+                        const int ey = el / B, ex = el - (el / B) * B;
+                        const int ty = qy2 + ey - y0, tx = qx2 + ex - x0;
+                        if (el < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
+                            const size_t off = (size_t)re * BB + el;
                             float* an = acc + ty * kAggStride + tx;
 #pragma unroll
                             for (int k = 0; k < NP; ++k) {
-                                an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + v[k][j];
-                                an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + w[k][j];
+                                an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + ld_global(c[k] + off);
+                                an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + ld_global(d[k] + rb);
                             }
                         }
                     }
                 }
+                continue;
             }
-        } else {
-            // B*B > 64: one record at a time, 64 elements per pass
-            while (m) {
-                const int l = __builtin_ctzll(m);
-                m &= m - 1;
-                const int rpk = __builtin_amdgcn_readlane(pk, l);
-                const int re = __builtin_amdgcn_readlane(e, l);
-                const int rb = __builtin_amdgcn_readlane(b, l);
-                const int qy2 = rpk >> 16, qx2 = rpk & 0xffff;
-                for (int c0 = 0; c0 < BB; c0 += 64) {
-                    const int el = c0 + lane;
-                    const int ey = el / B, ex = el - (el / B) * B;
-                    const int ty = qy2 + ey - y0, tx = qx2 + ex - x0;
-                    if (el < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
-                        const size_t off = (size_t)re * BB + el;
-                        float* an = acc + ty * kAggStride + tx;
-#pragma unroll
-                        for (int k = 0; k < NP; ++k) {
-                            an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + ld_global(c[k] + off);
-                            an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + ld_global(d[k] + rb);
-                        }
-                    }
-                }
+            const int cnt = __popcll(m);
+            if (cnt == 0) continue;
+            if (nrec + cnt > kAggRecCap) {
+                wave_sync_lds();
+                agg_consume_list<NP>(rec, nrec, B, lc, dy, dx, y0, x0, c, d, acc);
+                wave_sync_lds();
+                nrec = 0;
             }
+            if (ov) {
+                const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                int* e = rec + at * kAggRecWords;
+                e[0] = spk[u];
+                e[1] = sb[u] * G + sz[u];
+                e[2] = sb[u];
+            }
+            nrec += cnt;
         }
     }
+    wave_sync_lds();
+    agg_consume_list<NP>(rec, nrec, B, lc, dy, dx, y0, x0, c, d, acc);
     wave_sync_lds();
 }
 
 // Runs agg_tile_lds for the processed planes (k-th processed plane -> its
-// contribution and weight arrays, selected without indexing so the pointers
-// stay in registers); returns their number.
-__device__ __forceinline__ int agg_tile_any(const StageParams& p, const int* process, const int* __restrict__ count,
-                                            const int* __restrict__ pos, const int* __restrict__ frame, int fr,
-                                            int a_lo, int a_hi, int c_lo, int c_hi, int y0, int x0,
-                                            const float* const* contrib, const float* const* denw, float* acc) {
-    const bool p0 = process[0] != 0, p1 = process[1] != 0, p2 = process[2] != 0;
+// contribution and weight arrays, selected among values so that no
+// kernel-argument array is indexed at run time); returns their number.
+__device__ __forceinline__ int agg_tile_any(const StageParams& p, bool p0, bool p1, bool p2,
+                                            const int* __restrict__ count, const int* __restrict__ pos,
+                                            const int* __restrict__ frame, int fr, int a_lo, int a_hi, int c_lo,
+                                            int c_hi, int y0, int x0, const float* c0, const float* c1,
+                                            const float* c2, const float* d0, const float* d1, const float* d2,
+                                            float* acc, int* rec) {
     const int np = (int)p0 + (int)p1 + (int)p2;
-    const int i0 = p0 ? 0 : p1 ? 1 : 2;
-    const int i1 = (p0 && p1) ? 1 : 2;
     const float* c[3];
     const float* d[3];
-    c[0] = i0 == 0 ? contrib[0] : i0 == 1 ? contrib[1] : contrib[2];
-    d[0] = i0 == 0 ? denw[0] : i0 == 1 ? denw[1] : denw[2];
-    c[1] = i1 == 1 ? contrib[1] : contrib[2];
-    d[1] = i1 == 1 ? denw[1] : denw[2];
-    c[2] = contrib[2];
-    d[2] = denw[2];
-    if (np == 3) agg_tile_lds<3>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
-    else if (np == 2) agg_tile_lds<2>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
-    else if (np == 1) agg_tile_lds<1>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc);
+    c[0] = p0 ? c0 : (p1 ? c1 : c2);
+    d[0] = p0 ? d0 : (p1 ? d1 : d2);
+    c[1] = (p0 && p1) ? c1 : c2;
+    d[1] = (p0 && p1) ? d1 : d2;
+    c[2] = c2;
+    d[2] = d2;
+    if (np == 3) agg_tile_lds<3>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
+    else if (np == 2) agg_tile_lds<2>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
+    else if (np == 1) agg_tile_lds<1>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
     return np;
 }
 

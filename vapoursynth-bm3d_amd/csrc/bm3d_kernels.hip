@@ -1038,11 +1038,13 @@ hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order,
 // (BM3D_Base.cpp:455-469).
 
 // Writes the filtered float triple/single to the destination format.
-__device__ void write_output(const OutputSpec& o, int x, int y, const float* v, const bool* proc) {
+__device__ __forceinline__ void write_output(const OutputSpec& o, int x, int y, const float* v, const bool* proc) {
     if (o.mode == kOutFloat) {
+#pragma unroll
         for (int pl = 0; pl < 3; ++pl)
             if (proc[pl]) static_cast<float*>(o.dst[pl])[(size_t)y * o.pitch[pl] + x] = v[pl];
     } else if (o.mode == kOutInt) {
+#pragma unroll
         for (int pl = 0; pl < 3; ++pl) {
             if (!proc[pl]) continue;
             float t = v[pl] * o.gain[pl] + o.offset[pl];
@@ -1076,6 +1078,7 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
 
 __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
     __shared__ float acc[kAggLds];
+    __shared__ int rec[kAggRecCap * kAggRecWords];
     const int lane = lane_id();
     const int tiles_x = (a.p.width + kAggTW - 1) / kAggTW;
     const int tiles_y = (a.p.height + kAggTH - 1) / kAggTH;
@@ -1086,7 +1089,8 @@ __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
     int a_lo, a_hi, c_lo, c_hi;
     reach_rect(a.p, y0, x0, kAggTH, kAggTW, a.reach, a_lo, a_hi, c_lo, c_hi);
     bool proc[3] = {a.process[0] != 0, a.process[1] != 0, a.process[2] != 0};
-    agg_tile_any(a.p, a.process, a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib, a.denw, acc);
+    agg_tile_any(a.p, proc[0], proc[1], proc[2], a.count, a.pos, nullptr, 0, a_lo, a_hi, c_lo, c_hi, y0, x0,
+                 a.contrib[0], a.contrib[1], a.contrib[2], a.denw[0], a.denw[1], a.denw[2], acc, rec);
     for (int i = lane; i < kAggTH * kAggTW; i += 64) {
         const int ly = i / kAggTW, lx = i - ly * kAggTW;
         const int py = y0 + ly, px = x0 + lx;

@@ -266,6 +266,7 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 
 __global__ __launch_bounds__(64) void k_vstack(VStackArgs a) {
     __shared__ float acc[kAggLds];
+    __shared__ int rec[kAggRecCap * kAggRecWords];
     const int lane = lane_id();
     const int tiles_x = (a.p.width + kAggTW - 1) / kAggTW, tiles_y = (a.p.height + kAggTH - 1) / kAggTH;
     const int ntiles = tiles_x * tiles_y;
@@ -279,8 +280,9 @@ __global__ __launch_bounds__(64) void k_vstack(VStackArgs a) {
     const int reach = a.p.range + dist * (a.p.ps_range / a.p.ps_step * a.p.ps_step);
     int a_lo, a_hi, c_lo, c_hi;
     reach_rect(a.p, y0, x0, kAggTH, kAggTW, reach, a_lo, a_hi, c_lo, c_hi);
-    agg_tile_any(a.p, a.process, a.count, a.pos, a.frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib, a.denw,
-                 acc);
+    agg_tile_any(a.p, a.process[0] != 0, a.process[1] != 0, a.process[2] != 0, a.count, a.pos, a.frame, fr, a_lo,
+                 a_hi, c_lo, c_hi, y0, x0, a.contrib[0], a.contrib[1], a.contrib[2], a.denw[0], a.denw[1],
+                 a.denw[2], acc, rec);
     const int slot = 2 * (a.p.radius + fr - a.cur);
     for (int i = lane; i < kAggTH * kAggTW; i += 64) {
         const int ly = i / kAggTW, lx = i - ly * kAggTW;
