@@ -27,6 +27,8 @@ import numpy as np  # noqa: E402
 
 FRAMES, W, H, SIGMA = 8, 1920, 1080, 10.0
 PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
+PEAK_VALU_NONPACKED = 78.6  # FP32 VALU without packed math (FMA = 2 flops)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1080p_rgb48_np.json")
 METRIC = "Mpix/s BM3D Basic+Final 1080p RGB; learned-BM speedup at ≤0.05 dB PSNR loss"
 
 
@@ -45,6 +47,21 @@ def flops_per_frame(stats, B, wiener, planes=3):
     dct = planes * T * (4 * B ** 3 * stats["members"] + 2 * B * B * stats["members_sq"])
     shrink = planes * c * B * B * stats["members"]
     return ssd, dct + shrink
+
+
+def measured_traffic(timer):
+    """HBM bytes per launch of the kernel behind a timer, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this workload (calibrated by a
+    plain copy); None when that kernel was not profiled."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            d = json.load(fh)
+    except OSError:
+        return None
+    for k in d.get("kernels", {}).values():
+        if k.get("timer") == timer:
+            return k["hbm_bytes"]
+    return None
 
 
 def cpu_baseline(frames, rows, nthreads):
@@ -171,7 +188,8 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3) if achieved else None,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4) if achieved else None,
-                     "traffic": None,
+                     "traffic": measured_traffic(dom),
+                     "frac_of_nonpacked_valu": round(achieved / PEAK_VALU_NONPACKED, 4) if achieved else None,
                      "avg_launch_ms": round(avg_ms, 4),
                      "flops_per_launch": flops_launch},
         "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kern.items()},
