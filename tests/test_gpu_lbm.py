@@ -1,0 +1,134 @@
+"""GPU tests of the learned block-matching proposal.
+
+* the MFMA forward pass and the training gradients against the CPU oracle
+  (lbm_oracle, which replays the kernels' bf16 rounding points; only the
+  FP32 accumulation order differs, tolerance stated per check);
+* training on exhaustive match lists lowers the loss and raises recall over
+  the untrained network;
+* every learned match list is an exact re-rank: its keys are the plain
+  matcher's SSE2-order SSD keys of its positions, in (key, index) order, and
+  with the whole window proposed the lists and the filtered frame equal the
+  plain method's bit for bit."""
+import numpy as np
+import pytest
+
+import lbm
+import lbm_oracle as LO
+import oracle as O
+import synth
+
+pytestmark = pytest.mark.gpu
+
+DIST_MUL = 255.0 * 255.0 / 64.0
+
+
+def _plane(h, w, sigma, seed):
+    noisy, clean = synth.gray_float(h, w, sigma, seed=seed)
+    return noisy, clean
+
+
+def _rand_pairs(rng, h, w, n):
+    return np.stack([rng.randint(0, h - 7, n), rng.randint(0, w - 7, n),
+                     rng.randint(0, h - 7, n), rng.randint(0, w - 7, n)], axis=1).astype(np.int32)
+
+
+def test_forward_matches_oracle(gpu):
+    import torch
+    rng = np.random.RandomState(11)
+    noisy, _ = _plane(64, 80, 25.0, 900)
+    pairs = _rand_pairs(rng, 64, 80, 200)
+    model = lbm.Model(seed=5)
+    e = model.forward(torch.from_numpy(noisy).cuda(), torch.from_numpy(pairs).cuda())
+    _, eo = LO.forward(model.params(), LO.patches(noisy, pairs))
+    err = np.abs(e - eo).max()
+    assert err <= 1e-4 * max(1.0, np.abs(eo).max()), err
+
+
+def test_gradients_match_oracle(gpu):
+    import torch
+    rng = np.random.RandomState(12)
+    noisy, _ = _plane(72, 96, 25.0, 901)
+    pairs = _rand_pairs(rng, 72, 96, 300)
+    target = LO.pair_ssd(noisy, pairs).astype(np.float32)
+    model = lbm.Model(seed=6)
+    g, loss = model.gradients(torch.from_numpy(noisy).cuda(), torch.from_numpy(pairs).cuda(),
+                              torch.from_numpy(target).cuda())
+    lo, go = LO.loss_and_grad(model.params(), LO.patches(noisy, pairs), target)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    # per parameter block: |g - g_oracle| <= 2e-3 max|g_oracle| (FP32 sums of bf16 products)
+    for a, b in ((LO.OFF_W1, LO.OFF_B1), (LO.OFF_B1, LO.OFF_W2), (LO.OFF_W2, LO.OFF_B2), (LO.OFF_B2, LO.N_PARAMS)):
+        scale = np.abs(go[a:b]).max()
+        assert np.abs(g[a:b] - go[a:b]).max() <= 2e-3 * scale, (a, b)
+
+
+def _basic(bm3d, img, **kw):
+    return bm3d.Basic(bm3d.Clip.from_arrays(bm3d.GRAYS, [img[None]]), **kw)
+
+
+KW = dict(profile="fast", sigma=25.0, bm_range=12, group_size=16)
+
+
+def _exact_key(img, j, i, y, x):
+    rb = np.ascontiguousarray(img[j:j + 8, i:i + 8])
+    d = O.lib().or_ssd(O._fp(rb), O._fp(np.ascontiguousarray(img)), img.shape[1], y, x, 8, O.ARITH_SSE2)
+    return np.float32(np.float64(d) * DIST_MUL)
+
+
+def test_learned_lists_are_exact_reranks(gpu):
+    h, w = 96, 128
+    noisy, _ = _plane(h, w, 25.0, 902)
+    model = lbm.Model(seed=7)  # untrained: proposals are poor, the re-rank must still be exact
+    node = _basic(gpu, noisy, **KW)
+    node.filter.set_matcher(model, 32)
+    node.get_frame(0)
+    cnt, pos, key = node.filter.match_lists()
+    for b in range(0, len(cnt), 5):
+        n = int(cnt[b])
+        j, i = pos[b, 0]
+        assert key[b, 0] == 0.0
+        ks = key[b, 1:n]
+        assert np.all(np.diff(ks) >= 0), b
+        for z in range(1, n):
+            y, x = pos[b, z]
+            assert (y, x) != (j, i)
+            assert abs(int(y) - int(j)) <= 12 and abs(int(x) - int(i)) <= 12
+            assert key[b, z] == _exact_key(noisy, j, i, y, x), (b, z)
+
+
+def test_whole_window_proposals_equal_plain(gpu):
+    h, w = 64, 96
+    noisy, _ = _plane(h, w, 25.0, 903)
+    kw = dict(profile="fast", sigma=25.0, bm_range=3, group_size=16)  # 48 candidates <= top-k 64
+    plain = _basic(gpu, noisy, **kw)
+    out_p = np.asarray(plain.get_frame(0).planes[0])
+    learned = _basic(gpu, noisy, **kw)
+    learned.filter.set_matcher(lbm.Model(seed=8), 64)
+    out_l = np.asarray(learned.get_frame(0).planes[0])
+    cp, pp, kp = plain.filter.match_lists()
+    cl, pl, kl = learned.filter.match_lists()
+    assert np.array_equal(cp, cl)
+    for b in range(len(cp)):
+        n = cp[b]
+        assert np.array_equal(pp[b, :n], pl[b, :n]) and np.array_equal(kp[b, :n].view(np.uint32), kl[b, :n].view(np.uint32))
+    assert np.array_equal(out_p.view(np.uint32), out_l.view(np.uint32))
+
+
+def test_training_lowers_loss_and_raises_recall(gpu):
+    frames = [_plane(128, 128, 25.0, 910 + k)[0] for k in range(8)]
+    model = lbm.Model(seed=9)
+    untrained = lbm.Model(params=model.params())
+    hist = lbm.train(model, frames, KW, epochs=6, lr=3e-3, batch_pairs=4096, seed=1)
+    assert hist[-1] < 0.5 * hist[0], hist
+    held = _plane(128, 160, 25.0, 990)[0]
+    plain = _basic(gpu, held, **KW)
+    plain.get_frame(0)
+    cp, pp, _ = plain.filter.match_lists()
+    rec = {}
+    for name, m in (("trained", model), ("untrained", untrained)):
+        node = _basic(gpu, held, **KW)
+        node.filter.set_matcher(m, 32)
+        node.get_frame(0)
+        cl, pl, _ = node.filter.match_lists()
+        rec[name] = lbm.recall(cp, pp, cl, pl)
+    assert rec["trained"] > rec["untrained"] + 0.05, rec
+    assert rec["trained"] > 0.5, rec

@@ -17,7 +17,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared_functions():
-    text = open(os.path.join(ROOT, "include", "bm3d_amd.h")).read()
+    text = ""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text += open(os.path.join(ROOT, "include", h)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(bm3d_\w+)\s*\(", text, flags=re.M)
     return sorted(set(names))
@@ -26,7 +29,7 @@ def _declared_functions():
 def test_library_exports_declared_symbols():
     L = bm3d.load_library()
     names = _declared_functions()
-    assert "bm3d_create" in names and "bm3d_get_frame" in names
+    assert "bm3d_create" in names and "bm3d_get_frame" in names and "bm3d_lbm_train_step" in names
     for n in names:
         assert hasattr(L, n), n
     assert set(bm3d.EXPORTS) <= set(names)

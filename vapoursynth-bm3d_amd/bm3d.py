@@ -81,6 +81,13 @@ def load_library(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise Error(f"bm3d: HIP library not built ({path}); run vapoursynth-bm3d_amd/build.py")
+    # Device tensors handed over by torch must live in the HIP runtime the
+    # library runs on: load torch (and the runtime it brings) first, so that
+    # the library binds to that same runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     L.bm3d_init.argtypes = [C.c_int]
     L.bm3d_last_error.restype = C.c_char_p
@@ -97,6 +104,26 @@ def load_library(path: str = LIB_PATH):
                                     C.POINTER(C.c_int64)]
     L.bm3d_debug_matches.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int]
+    # learned block-matching proposal (bm3d_lbm.h)
+    L.bm3d_lbm_param_count.restype = C.c_int
+    L.bm3d_lbm_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_float)]
+    L.bm3d_lbm_free.argtypes = [C.c_void_p]
+    L.bm3d_lbm_free.restype = None
+    L.bm3d_lbm_get_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.bm3d_lbm_set_params.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+    L.bm3d_lbm_train_step.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_float,
+                                      C.POINTER(C.c_float)]
+    L.bm3d_lbm_gradients.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.bm3d_lbm_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+    L.bm3d_lbm_match_plane.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_float)]
+    L.bm3d_filter_params.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.bm3d_set_matcher.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    L.bm3d_matcher_fallbacks.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.bm3d_match_lists.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
     _lib = L
     return L
 
@@ -388,6 +415,42 @@ class _Filter:
             raise Error(self.L.bm3d_last_error().decode())
         n = cnt.value
         return np.array(ks[:n], np.float32), np.array(ys[:n], np.int32), np.array(xs[:n], np.int32)
+
+
+    # -- learned block-matching proposal (no counterpart in the reference)
+    def set_matcher(self, model, topk: int = 32):
+        """Basic/Final: propose candidates with a trained lbm.Model (None:
+        exhaustive search).  The cached frames of the clip are not cleared."""
+        self._matcher = model  # keep the model alive while attached
+        rc = self.L.bm3d_set_matcher(self.h, model.h if model is not None else None, topk)
+        if rc != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+
+    def params(self) -> Dict[str, int]:
+        """Basic/Final matching parameters after profile defaults."""
+        out = (C.c_int * 5)()
+        if self.L.bm3d_filter_params(self.h, out) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return dict(block_size=out[0], block_step=out[1], group_size=out[2], bm_range=out[3], bm_step=out[4])
+
+    def matcher_fallbacks(self) -> int:
+        n = C.c_int()
+        if self.L.bm3d_matcher_fallbacks(self.h, C.byref(n)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return n.value
+
+    def match_lists(self):
+        """count[nb], pos[nb, G] as (y, x) pairs, key[nb, G] of the last frame."""
+        nb, g = C.c_int64(), C.c_int()
+        self.L.bm3d_match_lists(self.h, C.byref(nb), C.byref(g), None, None, None)
+        cnt = np.zeros(nb.value, np.int32)
+        pos = np.zeros((nb.value, g.value), np.int32)
+        key = np.zeros((nb.value, g.value), np.float32)
+        rc = self.L.bm3d_match_lists(self.h, C.byref(nb), C.byref(g), cnt.ctypes.data, pos.ctypes.data,
+                                     key.ctypes.data)
+        if rc != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return cnt, np.stack([pos >> 16, pos & 0xFFFF], axis=-1), key
 
 
 def _node(name: str, clip: Clip, ref: Optional[Clip], kwargs) -> Clip:

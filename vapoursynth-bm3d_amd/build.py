@@ -20,7 +20,7 @@ LIB = os.path.join(OUT_DIR, "libbm3d_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BM3D_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bm3d_kernels.hip", "bm3d_vkernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp"]
+SOURCES = ["bm3d_kernels.hip", "bm3d_vkernels.hip", "lbm_kernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp", "lbm.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -50,7 +50,14 @@ struct DevBuf {
     }
 };
 
+// Work buffers of the learned proposal path of one filter.
+struct LbmWork {
+    DevBuf emb, norm, scores, fallback, nfallback;
+};
+
 }  // namespace bm3d
+
+struct bm3d_lbm;
 
 struct bm3d_filter {
     bm3d::Fn fn;
@@ -74,6 +81,10 @@ struct bm3d_filter {
     bm3d::DevBuf fplane[2][3 * 33];  // converted float planes
     bm3d::DevBuf out_stage[3];
     bm3d::DevBuf m_count, m_pos, m_key, m_frame, m_order, m_hist, contrib[3], denw[3], ptrs;
+    // learned block-matching proposal (null: exhaustive search)
+    bm3d_lbm* lbm = nullptr;
+    int lbm_topk = 32;
+    bm3d::LbmWork lbm_work;
     hipStream_t stream = nullptr;
     std::mutex mu;
     bm3d_stats stats{};
@@ -101,4 +112,8 @@ enum Timer { kTMatch = 0, kTFilter = 1, kTAggregate = 2, kTConvert = 3, kTVAggre
 void prof_begin(bm3d_filter* f, int t, const char* name);
 void prof_end(bm3d_filter* f, int t);
 void prof_collect(bm3d_filter* f);
+StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch);
+// Block matching of one plane (exhaustive, or the proposal path of m).
+int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const StageParams& sp, int* count,
+              int* pos, float* key, hipStream_t st);
 }

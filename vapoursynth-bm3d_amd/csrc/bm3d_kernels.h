@@ -117,7 +117,7 @@ hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order,
 bool filter_uses_buckets(int B, int G);
 bool verify_dct_constants();
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
-                        float* key, hipStream_t st);
+                        float* key, hipStream_t st, const int* blist = nullptr, const int* nlist = nullptr);
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st);
 hipError_t launch_aggregate(const AggArgs& a, hipStream_t st);
 hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,

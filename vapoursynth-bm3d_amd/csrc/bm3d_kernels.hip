@@ -150,15 +150,23 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
 // keys in registers.
 constexpr int kSelCap = 256;
 
-template <int KMAX, int W>
-__global__ __launch_bounds__(64 * kMatchWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_match8(const float* __restrict__ plane, StageParams p, int nb,
-                                                             int wave_words, int* __restrict__ out_count,
-                                                             int* __restrict__ out_pos, float* __restrict__ out_key) {
+// NW waves (reference blocks) per workgroup.  With blist, wave w of the grid
+// takes reference block blist[w] for w < *nlist (a sparse re-run).
+template <int KMAX, int W, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8(
+    const float* __restrict__ plane, StageParams p, int nb, int wave_words, const int* __restrict__ blist,
+    const int* __restrict__ nlist, int* __restrict__ out_count, int* __restrict__ out_pos,
+    float* __restrict__ out_key) {
     extern __shared__ uint32_t smem[];
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
-    const int b = blockIdx.x * kMatchWaves + wave;
-    if (b >= nb) return;
+    int b = blockIdx.x * NW + wave;
+    if (blist) {
+        if (b >= uniform(*nlist)) return;
+        b = uniform(blist[b]);
+    } else if (b >= nb) {
+        return;
+    }
     constexpr int B = 8;
     constexpr int NK = KMAX * W;
     const int G = p.group;
@@ -1204,31 +1212,34 @@ Match8Shape match8_shape(int max_cand, int G) {
 }
 
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
-                        float* key, hipStream_t st) {
-    int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
-    if (blocks == 0) return hipSuccess;
+                        float* key, hipStream_t st, const int* blist, const int* nlist) {
+    if (nb <= 0) return hipSuccess;
     if (p.block == 8 && p.step == 1) {
         const Match8Shape sh = match8_shape(max_cand, p.group);
-        const size_t lds = sh.words * 4 * kMatchWaves;
         const int km = sh.kmax <= 1 ? 1 : sh.kmax <= 2 ? 2 : sh.kmax <= 3 ? 3 : sh.kmax <= 4 ? 4 : 0;
-        if (km != 0 && lds <= 65536) {
-#define BM3D_M8(K, WD)                                                                                        \
-    if (km == K && sh.w == WD) {                                                                              \
-        hipLaunchKernelGGL((k_match8<K, WD>), dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, nb,    \
-                           (int)sh.words, count, pos, key);                                                   \
-        return hipGetLastError();                                                                             \
+        const size_t wbytes = sh.words * 4;
+        const int nw = wbytes * 4 <= 65536 ? 4 : wbytes * 2 <= 65536 ? 2 : wbytes <= 65536 ? 1 : 0;
+        if (km != 0 && nw != 0) {
+            const int blocks = (nb + nw - 1) / nw;
+            const size_t lds = wbytes * nw;
+#define BM3D_M8(K, WD, NW)                                                                                   \
+    if (km == K && sh.w == WD && nw == NW) {                                                                 \
+        hipLaunchKernelGGL((k_match8<K, WD, NW>), dim3(blocks), dim3(64 * NW), lds, st, plane, p, nb,        \
+                           (int)sh.words, blist, nlist, count, pos, key);                                    \
+        return hipGetLastError();                                                                            \
     }
-            BM3D_M8(1, 8)
-            BM3D_M8(2, 8)
-            BM3D_M8(3, 8)
-            BM3D_M8(4, 8)
-            BM3D_M8(1, 11)
-            BM3D_M8(2, 11)
-            BM3D_M8(3, 11)
-            BM3D_M8(4, 11)
+#define BM3D_M8W(NW) \
+    BM3D_M8(1, 8, NW) BM3D_M8(2, 8, NW) BM3D_M8(3, 8, NW) BM3D_M8(4, 8, NW) \
+    BM3D_M8(1, 11, NW) BM3D_M8(2, 11, NW) BM3D_M8(3, 11, NW) BM3D_M8(4, 11, NW)
+            BM3D_M8W(4)
+            BM3D_M8W(2)
+            BM3D_M8W(1)
+#undef BM3D_M8W
 #undef BM3D_M8
         }
     }
+    if (blist) return hipErrorInvalidValue;  // sparse re-runs need the B = 8 fast path
+    int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
     size_t lds = match_lds_bytes(max_cand, p.block, p.group);
     int ww = (int)match_wave_words(max_cand, p.block, p.group);
     if (p.block == 8)

@@ -15,6 +15,43 @@
 
 namespace bm3d {
 
+// Builds the stage parameters of a Basic/Final filter for a W x H plane.
+StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
+    StageParams p{};
+    p.width = W;
+    p.height = H;
+    p.pitch = (int)pitch;
+    p.block = f->B;
+    p.block_step = f->step;
+    p.group = f->G;
+    p.step = f->S;
+    p.range = f->R / f->S * f->S;
+    auto count = [](int dim, int B, int s) {
+        int last = dim - B, n = 0;
+        for (int j = 0;; j += s) {
+            if (j >= last + s) break;
+            if (j > last) j = last;
+            ++n;
+        }
+        return n;
+    };
+    p.nx = count(W, f->B, f->step);
+    p.ny = count(H, f->B, f->step);
+    // Block::BlockMatchingMulti (include/Block.h:558-560)
+    const double mse2sse = (double)(f->B * f->B) * 1.0f * 1.0f / (double)(255 * 255);
+    p.dist_mul = 1.0 / mse2sse;
+    p.th_sse = (float)(f->th_mse * mse2sse);
+    if (f->G == 1 || f->th_mse <= 0) {
+        p.th_sse = 0.0f;  // kernel takes the short-cut
+        p.dist_mul = 0.0;
+    }
+    p.radius = f->radius;
+    p.ps_num = f->ps_num;
+    p.ps_range = f->ps_range;
+    p.ps_step = f->ps_step;
+    return p;
+}
+
 namespace {
 
 constexpr int kPitchAlign = 64;  // floats: 256-byte aligned rows
@@ -230,42 +267,6 @@ int run_color(bm3d_filter* f, const bm3d_frame& s, bm3d_frame& d) {
     return BM3D_OK;
 }
 
-// Builds the stage parameters of a Basic/Final filter for a W x H plane.
-StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
-    StageParams p{};
-    p.width = W;
-    p.height = H;
-    p.pitch = (int)pitch;
-    p.block = f->B;
-    p.block_step = f->step;
-    p.group = f->G;
-    p.step = f->S;
-    p.range = f->R / f->S * f->S;
-    auto count = [](int dim, int B, int s) {
-        int last = dim - B, n = 0;
-        for (int j = 0;; j += s) {
-            if (j >= last + s) break;
-            if (j > last) j = last;
-            ++n;
-        }
-        return n;
-    };
-    p.nx = count(W, f->B, f->step);
-    p.ny = count(H, f->B, f->step);
-    // Block::BlockMatchingMulti (include/Block.h:558-560)
-    const double mse2sse = (double)(f->B * f->B) * 1.0f * 1.0f / (double)(255 * 255);
-    p.dist_mul = 1.0 / mse2sse;
-    p.th_sse = (float)(f->th_mse * mse2sse);
-    if (f->G == 1 || f->th_mse <= 0) {
-        p.th_sse = 0.0f;  // kernel takes the short-cut
-        p.dist_mul = 0.0;
-    }
-    p.radius = f->radius;
-    p.ps_num = f->ps_num;
-    p.ps_range = f->ps_range;
-    p.ps_step = f->ps_step;
-    return p;
-}
 
 // Exact number of SSD evaluations of a frame: every lattice position of
 // every reference block's search window except the block itself
@@ -380,7 +381,14 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     if (match_lds_bytes(max_cand, B, G) > 65536)
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the matching kernel");
     prof_begin(f, kTMatch, f->wiener ? "k_match(final)" : "k_match(basic)");
-    HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, f->m_count.as<int>(), f->m_pos.as<int>(), f->m_key.as<float>(), st));
+    if (f->lbm) {
+        if ((rc = lbm_match(f->lbm, f->lbm_topk, f->lbm_work, Rf[0], sp, f->m_count.as<int>(), f->m_pos.as<int>(),
+                            f->m_key.as<float>(), st)))
+            return rc;
+    } else {
+        HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, f->m_count.as<int>(), f->m_pos.as<int>(),
+                             f->m_key.as<float>(), st));
+    }
     prof_end(f, kTMatch);
     f->last_nb = nb;
 
