@@ -122,7 +122,10 @@ int bm3d_frame_range(const bm3d_filter *f, int n, int *first, int *last);
  * are provided by the caller (host or device memory); dst->props receives
  * the output frame's properties.  Replaces the *_GetFrame arAllFramesReady
  * path (VSProcess::process, include/Helper.h:695-725, and the
- * process_core / Kernel functions behind it). */
+ * process_core / Kernel functions behind it).  Runs on the filter's stream,
+ * which is ordered after the legacy default stream: device planes written on
+ * another stream must be complete before the call.  Returns when dst is
+ * written. */
 int bm3d_get_frame(bm3d_filter *f, int n, const bm3d_frame *src, const bm3d_frame *ref, bm3d_frame *dst);
 
 /* Releases the filter (the *_Free callbacks, source/VSPlugin.cpp:80-84 etc.). */

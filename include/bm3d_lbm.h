@@ -12,7 +12,9 @@
  *
  * All plane, pair and match-list pointers are device pointers unless stated.
  * Planes are float32 with a row pitch in elements.  Calls are serialised on
- * the model's (or the filter's) own HIP stream and return when done.
+ * the model's (or the filter's) own HIP stream and return when done.  That
+ * stream is ordered after the legacy default stream only: device inputs
+ * written on another stream must be complete before the call.
  */
 #ifndef BM3D_LBM_H
 #define BM3D_LBM_H

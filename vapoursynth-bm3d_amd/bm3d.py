@@ -271,6 +271,14 @@ def _plane_ptr(a):
     return a.data_ptr(), a.stride(0) * a.element_size(), 1
 
 
+def _sync_producer(device) -> None:
+    """Waits for torch's current stream on device: the library's streams are
+    ordered only after the legacy default stream, so device inputs produced on
+    another torch stream must be complete before they are handed over."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def _frame_to_c(fr: Frame, keep: list) -> FrameC:
     c = FrameC()
     mem = None
@@ -376,6 +384,8 @@ class _Filter:
                 refs[i] = _frame_to_c(self.ref.get_frame(m), keep)
         src0 = self.clip.get_frame(n)
         device = None if not src0.on_device else src0.planes[0].device
+        if device is not None:
+            _sync_producer(device)
         planes = _alloc_like(self.out_fmt, self.out_w, self.out_h, device)
         out = Frame(planes, {})
         dst = _frame_to_c(out, keep)

@@ -547,7 +547,9 @@ int bm3d_create(const char* function, const bm3d_clip_info* input, const bm3d_cl
         build_tables(f.get(), sig);
         if ((rc = upload_coefs(f.get()))) return rc;
     }
-    HIP_TRY(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+    // blocking stream: ordered after work on the legacy default stream (where
+    // callers such as torch produce device inputs) and before later work on it
+    HIP_TRY(hipStreamCreate(&f->stream));
     *out = f.release();
     return BM3D_OK;
 }

@@ -123,33 +123,25 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     int rc;
     if ((rc = w.emb.ensure(npos * L::kEmb * sizeof(_Float16)))) return rc;
     if ((rc = w.norm.ensure(npos * sizeof(float)))) return rc;
-    if ((rc = w.scores.ensure((size_t)nb * max_cand * sizeof(_Float16)))) return rc;
     if ((rc = w.fallback.ensure((size_t)nb * sizeof(int)))) return rc;
     if ((rc = w.nfallback.ensure(sizeof(int)))) return rc;
     const L::Net n = m->net();
     HIP_TRY(L::launch_embed(n, plane, sp.pitch, W, H, w.emb.as<_Float16>(), w.norm.as<float>(), st));
-    L::ScoreArgs sa{};
-    sa.p = sp;
-    sa.emb = w.emb.as<_Float16>();
-    sa.norm = w.norm.as<float>();
-    sa.scores = w.scores.as<_Float16>();
-    sa.max_cand = max_cand;
-    HIP_TRY(L::launch_score(sa, st));
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, w.nfallback.as<int>());
     HIP_TRY(hipGetLastError());
-    L::SelectArgs se{};
-    se.p = sp;
-    se.nb = nb;
-    se.plane = plane;
-    se.scores = w.scores.as<_Float16>();
-    se.max_cand = max_cand;
-    se.topk = topk;
-    se.count = count;
-    se.pos = pos;
-    se.key = key;
-    se.fallback = w.fallback.as<int>();
-    se.nfallback = w.nfallback.as<int>();
-    HIP_TRY(L::launch_select(se, st));
+    L::ProposeArgs pa{};
+    pa.p = sp;
+    pa.plane = plane;
+    pa.emb = w.emb.as<_Float16>();
+    pa.norm = w.norm.as<float>();
+    pa.max_cand = max_cand;
+    pa.topk = topk;
+    pa.count = count;
+    pa.pos = pos;
+    pa.key = key;
+    pa.fallback = w.fallback.as<int>();
+    pa.nfallback = w.nfallback.as<int>();
+    HIP_TRY(L::launch_propose(pa, st));
     HIP_TRY(launch_match(plane, sp, nb, max_cand, count, pos, key, st, w.fallback.as<int>(), w.nfallback.as<int>()));
     return BM3D_OK;
 }
@@ -182,7 +174,7 @@ int bm3d_lbm_create(bm3d_lbm** out, const float* params) {
         delete m;
         return r;
     };
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreate(&m->stream) != hipSuccess)  // ordered with the legacy default stream
         return fail(set_error(BM3D_E_HIP, "bm3d_lbm_create: stream creation failed"));
     if ((rc = m->master.ensure(L::kParams * sizeof(float)))) return fail(rc);
     if ((rc = m->w1h.ensure(L::kHidden * L::kIn * sizeof(short)))) return fail(rc);

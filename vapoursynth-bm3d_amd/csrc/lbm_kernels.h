@@ -48,20 +48,12 @@ struct TrainBatch {
     float* dhs;           // [2 npairs][kHidden] dL/dH before the ReLU
 };
 
-struct ScoreArgs {
+struct ProposeArgs {
     StageParams p;
+    const float* plane;
     const _Float16* emb;  // [(H-7) * (W-7)][kEmb]
     const float* norm;    // [(H-7) * (W-7)]
-    _Float16* scores;     // [nb][max_cand]
-    int max_cand;
-};
-
-struct SelectArgs {
-    StageParams p;
-    int nb;
-    const float* plane;
-    const _Float16* scores;
-    int max_cand;
+    int max_cand;         // (2 range + 1)^2
     int topk;
     int* count;
     int* pos;
@@ -81,9 +73,8 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
                            hipStream_t st);
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st);
-size_t score_lds_bytes(const StageParams& p);
-hipError_t launch_score(const ScoreArgs& a, hipStream_t st);
-hipError_t launch_select(const SelectArgs& a, hipStream_t st);
+size_t propose_lds_bytes(int max_cand);
+hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
 
 }  // namespace lbm
 }  // namespace bm3d

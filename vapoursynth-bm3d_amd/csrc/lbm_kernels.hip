@@ -29,13 +29,10 @@ namespace lbm {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ short f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
+// float -> bfloat16 bits, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
@@ -84,12 +81,15 @@ __device__ __forceinline__ s4 patch_frag(const float* __restrict__ plane, int pi
                                          float* raw) {
     const int q = lane_id() >> 4;
     const float* p = plane + (size_t)(py + 2 * s + (q >> 1)) * pitch + px + 4 * (q & 1);
+    float4 f;
+    __builtin_memcpy(&f, p, sizeof(f));  // four consecutive samples (4-byte aligned)
+    raw[0] = f.x;
+    raw[1] = f.y;
+    raw[2] = f.z;
+    raw[3] = f.w;
     s4 v;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        raw[i] = p[i];
-        v[i] = f2bf(raw[i]);
-    }
+    for (int i = 0; i < 4; ++i) v[i] = f2bf(raw[i]);
     return v;
 }
 
@@ -99,18 +99,15 @@ __device__ __forceinline__ s4 patch_frag(const float* __restrict__ plane, int pi
 __device__ __forceinline__ void forward16(const Frags& f, const s4 (&x)[4], float (&h)[8][4], float (&e)[2][4]) {
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        f4 acc = {f.b1[t][0], f.b1[t][1], f.b1[t][2], f.b1[t][3]};  // bias as the initial sum
 #pragma unroll
         for (int s = 0; s < 4; ++s) acc = mfma(f.w1[t][s], x[s], acc);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float v = acc[i] + f.b1[t][i];
-            h[t][i] = v > 0.0f ? v : 0.0f;
-        }
+        for (int i = 0; i < 4; ++i) h[t][i] = acc[i] > 0.0f ? acc[i] : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-        f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        f4 acc = {f.b2[u][0], f.b2[u][1], f.b2[u][2], f.b2[u][3]};
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             s4 hb;
@@ -119,7 +116,7 @@ __device__ __forceinline__ void forward16(const Frags& f, const s4 (&x)[4], floa
             acc = mfma(f.w2[u][s], hb, acc);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) e[u][i] = acc[i] + f.b2[u][i];
+        for (int i = 0; i < 4; ++i) e[u][i] = acc[i];
     }
 }
 
@@ -404,253 +401,290 @@ __global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// inference: scores of every candidate of every reference block
+// inference: proposals, exact re-rank and fallback list
 // ---------------------------------------------------------------------------
 //
-// score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c (clamped at 0), fp16, stored at
-// scores[b * max_cand + cy * ncols + cx] in the window's row-major lattice
-// order (the self position included; it is skipped by the selection).  A
-// workgroup takes kTileRY x kTileRX reference blocks; the union of their
-// windows is streamed through LDS kChunkRows rows at a time and every thread
-// scores its union positions against every reference block of the tile
-// whose window contains them.
-constexpr int kTileRY = 4, kTileRX = 8, kTileRefs = kTileRY * kTileRX;
-constexpr int kChunkRows = 8;
-
-__device__ __forceinline__ float dot32(const uint32_t* a, const uint4 (&b)[4]) {
-    float acc = 0.0f;
-    const uint32_t bw[16] = {b[0].x, b[0].y, b[0].z, b[0].w, b[1].x, b[1].y, b[1].z, b[1].w,
-                             b[2].x, b[2].y, b[2].z, b[2].w, b[3].x, b[3].y, b[3].z, b[3].w};
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-        acc = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, a[k]), __builtin_bit_cast(h2, bw[k]), acc, false);
-    return acc;
-}
-
-__global__ __launch_bounds__(256) void k_score(ScoreArgs a) {
-    extern __shared__ uint4 lds4[];
-    __shared__ uint32_t eref[kTileRefs][16];
-    __shared__ float nref[kTileRefs];
-    __shared__ int win[kTileRefs][5];  // t, l, nrows, ncols, valid
-    const StageParams& p = a.p;
-    const int tiles_x = (p.nx + kTileRX - 1) / kTileRX;
-    const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - ty * tiles_x;
-    const int ew = p.width - 7;
-    const int B = 8;
-    // tile geometry and the union of the windows
-    int u_t = 1 << 30, u_b = -1, u_l = 1 << 30, u_r = -1;
-    for (int k = 0; k < kTileRefs; ++k) {
-        const int bj = ty * kTileRY + k / kTileRX, bi = tx * kTileRX + k % kTileRX;
-        if (bj >= p.ny || bi >= p.nx) continue;
-        const int j = grid_pos(bj, p.block_step, p.height - B), i = grid_pos(bi, p.block_step, p.width - B);
-        const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
-        const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
-        u_t = t < u_t ? t : u_t;
-        u_b = bt > u_b ? bt : u_b;
-        u_l = l < u_l ? l : u_l;
-        u_r = rr > u_r ? rr : u_r;
-    }
-    const int tid = threadIdx.x;
-    if (tid < kTileRefs) {
-        const int k = tid;
-        const int bj = ty * kTileRY + k / kTileRX, bi = tx * kTileRX + k % kTileRX;
-        const bool ok = bj < p.ny && bi < p.nx;
-        int j = 0, i = 0;
-        if (ok) {
-            j = grid_pos(bj, p.block_step, p.height - B);
-            i = grid_pos(bi, p.block_step, p.width - B);
-            const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
-            const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
-            win[k][0] = t;
-            win[k][1] = l;
-            win[k][2] = bt - t + 1;
-            win[k][3] = rr - l + 1;
-        } else {
-            win[k][0] = win[k][1] = win[k][2] = win[k][3] = 0;
-        }
-        win[k][4] = ok;
-        const size_t e = (size_t)j * ew + i;
-        const uint4* src = reinterpret_cast<const uint4*>(a.emb + e * kEmb);
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const uint4 v = src[w];
-            eref[k][4 * w + 0] = v.x;
-            eref[k][4 * w + 1] = v.y;
-            eref[k][4 * w + 2] = v.z;
-            eref[k][4 * w + 3] = v.w;
-        }
-        nref[k] = a.norm[e];
-    }
-    const int ucols = u_r - u_l + 1;
-    for (int c0 = u_t; c0 <= u_b; c0 += kChunkRows) {
-        const int rows = (u_b - c0 + 1) < kChunkRows ? (u_b - c0 + 1) : kChunkRows;
-        __syncthreads();
-        for (int e = tid; e < rows * ucols * 4; e += 256) {
-            const int pos = e >> 2, w = e & 3;
-            const int yy = c0 + pos / ucols, xx = u_l + pos % ucols;
-            lds4[e] = reinterpret_cast<const uint4*>(a.emb + ((size_t)yy * ew + xx) * kEmb)[w];
-        }
-        __syncthreads();
-        for (int pos = tid; pos < rows * ucols; pos += 256) {
-            const int yy = c0 + pos / ucols, xx = u_l + pos % ucols;
-            uint4 eq[4];
-#pragma unroll
-            for (int w = 0; w < 4; ++w) eq[w] = lds4[pos * 4 + w];
-            const float nq = a.norm[(size_t)yy * ew + xx];
-            for (int k = 0; k < kTileRefs; ++k) {
-                const int t = win[k][0], l = win[k][1];
-                const int cy = yy - t, cx = xx - l;
-                if (!win[k][4] || cy < 0 || cy >= win[k][2] || cx < 0 || cx >= win[k][3]) continue;
-                float d = nref[k] + nq - 2.0f * dot32(eref[k], eq);
-                d = d > 0.0f ? d : 0.0f;
-                const int bj = ty * kTileRY + k / kTileRX, bi = tx * kTileRX + k % kTileRX;
-                const size_t b = (size_t)bj * p.nx + bi;
-                a.scores[b * a.max_cand + cy * win[k][3] + cx] = (_Float16)d;
-            }
-        }
-    }
-}
-
-size_t score_lds_bytes(const StageParams& p) {
-    const int ucols = (kTileRX - 1) * p.block_step + 2 * p.range + 1;
-    return (size_t)kChunkRows * ucols * 4 * sizeof(uint4);
-}
-
-// ---------------------------------------------------------------------------
-// inference: proposal selection and exact re-rank
-// ---------------------------------------------------------------------------
+// One workgroup (four waves) per row of 16 reference blocks.
 //
-// One wave per reference block.  Keys (fp16 score bits << 16 | candidate
+// Scores: score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c (clamped at 0) of every
+// candidate c of each block's search window, as fp16 in LDS at
+// [block][cy * ncols + cx] (window row-major).  The dot products of the 16
+// blocks with 16 consecutive positions of a union row are one 16x16x32
+// product (two v_mfma_f32_16x16x16_f16), so every candidate embedding is read
+// once per workgroup; the waves take interleaved union rows.
+//
+// Proposals (one wave per block): keys (fp16 score bits << 16 | candidate
 // index) are unique, so the K smallest are the K best scores with ties going
-// to the earlier candidate.  The K proposals are then scored exactly like the
-// plain matcher (ssd_fixed<8>, dist <= thSSE && dist != 0, key =
-// float(double(dist) * distMul)) and the G-1 best by (key, candidate index)
-// form the group after the self entry.  A block whose window holds no more
-// than K candidates is searched exhaustively here; a block with fewer than
-// G-1 accepted proposals is appended to the fallback list.
-__global__ __launch_bounds__(256) void k_select(SelectArgs a) {
-    __shared__ uint32_t sk[4][kMaxProposals];
-    __shared__ uint32_t ek[4][kMaxProposals];
+// to the earlier candidate.  The K-th smallest lane minimum t0 bounds the
+// K-th smallest key (K lane minima are <= t0); the keys <= t0 (usually a few
+// K) are compacted and the exact K-th smallest is found among them.
+//
+// Re-rank: the K proposals are scored exactly like the plain matcher
+// (SSE2 lane-order SSD, dist <= thSSE && dist != 0, key = float(double(dist)
+// * distMul)) and the G-1 best by (key, candidate index) follow the self
+// entry.  A block whose window holds no more than K candidates is searched
+// exhaustively here; a block with fewer than G-1 accepted proposals is
+// appended to the fallback list.
+constexpr int kPropRefs = 16;
+constexpr int kSelBuf = 256;
+
+// LDS stride (fp16 elements) of one block's scores: room for the whole window
+// and for the wave's selection scratch (kMaxProposals + kSelBuf words) that
+// later aliases it; even, so the scratch is 4-byte aligned.
+__host__ __device__ __forceinline__ int prop_stride(int max_cand) {
+    const int need = 2 * (kMaxProposals + kSelBuf);
+    return ((max_cand > need ? max_cand : need) + 1) & ~1;
+}
+
+__device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// SSD of the 8x8 block rb (row-major registers) against the block at q, in
+// the SSE2 lane order of ssd_fixed<8>.
+__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch) {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        float4 lo, hi;
+        __builtin_memcpy(&lo, q + r * pitch, sizeof(lo));
+        __builtin_memcpy(&hi, q + r * pitch + 4, sizeof(hi));
+        const float d0 = rb[r * 8 + 0] - lo.x, d1 = rb[r * 8 + 1] - lo.y;
+        const float d2 = rb[r * 8 + 2] - lo.z, d3 = rb[r * 8 + 3] - lo.w;
+        s0 = s0 + d0 * d0;
+        s1 = s1 + d1 * d1;
+        s2 = s2 + d2 * d2;
+        s3 = s3 + d3 * d3;
+        const float e0 = rb[r * 8 + 4] - hi.x, e1 = rb[r * 8 + 5] - hi.y;
+        const float e2 = rb[r * 8 + 6] - hi.z, e3 = rb[r * 8 + 7] - hi.w;
+        s0 = s0 + e0 * e0;
+        s1 = s1 + e1 * e1;
+        s2 = s2 + e2 * e2;
+        s3 = s3 + e3 * e3;
+    }
+    return 0.0f + (((s0 + s1) + s2) + s3);
+}
+
+__global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
+    extern __shared__ unsigned short sc_lds[];  // [kPropRefs][prop_stride] fp16 scores
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
-    const int b = uniform((int)(blockIdx.x * 4 + wave));
-    if (b >= a.nb) return;
-    constexpr int B = 8;
+    const int r = lane & 15, q = lane >> 4;
+    const int col_tiles = (p.nx + kPropRefs - 1) / kPropRefs;
+    const int bj = blockIdx.x / col_tiles;
+    const int bi0 = (blockIdx.x - bj * col_tiles) * kPropRefs;
+    const int ew = p.width - 7;
+    const int B = 8;
     const int G = p.group;
     const int K = a.topk;
-    const int bj = b / p.nx, bi = b - bj * p.nx;
-    const int j = grid_pos(bj, p.block_step, p.height - B), i = grid_pos(bi, p.block_step, p.width - B);
-    const int l = search_lo(i, p.range, 1), r = search_hi(i, p.width - B, p.range, 1);
+    const int last_bi = bi0 + kPropRefs - 1 < p.nx - 1 ? bi0 + kPropRefs - 1 : p.nx - 1;
+    const int j = grid_pos(bj, p.block_step, p.height - B);
     const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
-    const int ncols = r - l + 1, nrows = bt - t + 1, nc = ncols * nrows;
-    const int self = (j - t) * ncols + (i - l);
-    int* cnt_out = a.count + b;
-    int* pos_out = a.pos + (size_t)b * G;
-    float* key_out = a.key + (size_t)b * G;
-    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
-        if (lane == 0) {
-            *cnt_out = 1;
-            pos_out[0] = (j << 16) | i;
-            key_out[0] = 0.0f;
+    const int nrows = bt - t + 1;
+    const bool shortcut = G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0);
+    const int rs = prop_stride(a.max_cand);
+
+    // ---- phase 1: window scores into LDS
+    if (!shortcut) {
+        const int ia = grid_pos(bi0 + r <= last_bi ? bi0 + r : last_bi, p.block_step, p.width - B);
+        int lcol[4], ncol[4];
+        bool cok[4];
+        float nref[4];
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii) {
+            const int bi = bi0 + 4 * q + ii;
+            cok[ii] = bi <= last_bi;
+            const int i = grid_pos(cok[ii] ? bi : last_bi, p.block_step, p.width - B);
+            lcol[ii] = search_lo(i, p.range, 1);
+            ncol[ii] = search_hi(i, p.width - B, p.range, 1) - lcol[ii] + 1;
+            nref[ii] = a.norm[(size_t)j * ew + i];
         }
-        return;
+        const int u_l = search_lo(grid_pos(bi0, p.block_step, p.width - B), p.range, 1);
+        const int u_r = search_hi(grid_pos(last_bi, p.block_step, p.width - B), p.width - B, p.range, 1);
+        h4 af0, af1;
+        {
+            const _Float16* er = a.emb + ((size_t)j * ew + ia) * kEmb;
+            __builtin_memcpy(&af0, er + 4 * q, sizeof(af0));
+            __builtin_memcpy(&af1, er + 16 + 4 * q, sizeof(af1));
+        }
+        const int ncb = (u_r - u_l + 16) >> 4;
+        const int ntile = nrows * ncb;
+        // tiles (union row, column block) dealt to the waves; B fragments one tile ahead
+        int tl = wave;
+        h4 b0 = {}, b1 = {};
+        float nq = 0.0f;
+        auto load_tile = [&](int tt, h4& x0f, h4& x1f, float& nqf) {
+            const int yy = t + tt / ncb;
+            const int xx = u_l + (tt - (tt / ncb) * ncb) * 16 + r;
+            const int xl = xx <= u_r ? xx : u_r;
+            const _Float16* ec = a.emb + ((size_t)yy * ew + xl) * kEmb;
+            __builtin_memcpy(&x0f, ec + 4 * q, sizeof(x0f));
+            __builtin_memcpy(&x1f, ec + 16 + 4 * q, sizeof(x1f));
+            nqf = a.norm[(size_t)yy * ew + xl];
+        };
+        if (tl < ntile) load_tile(tl, b0, b1, nq);
+        for (; tl < ntile; tl += 4) {
+            h4 c0 = b0, c1 = b1;
+            const float cq = nq;
+            if (tl + 4 < ntile) load_tile(tl + 4, b0, b1, nq);
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma_h(af0, c0, acc);
+            acc = mfma_h(af1, c1, acc);
+            const int cy = tl / ncb;
+            const int x = u_l + (tl - cy * ncb) * 16 + r;
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int cx = x - lcol[ii];
+                if (!cok[ii] || x > u_r || cx < 0 || cx >= ncol[ii]) continue;
+                float d = nref[ii] + cq - 2.0f * acc[ii];
+                d = d > 0.0f ? d : 0.0f;
+                sc_lds[(4 * q + ii) * rs + cy * ncol[ii] + cx] = __builtin_bit_cast(unsigned short, (_Float16)d);
+            }
+        }
     }
-    // --- proposals: the K smallest keys among the nc - 1 candidates
-    uint32_t* props = sk[wave];
-    int nprop;
-    if (nc - 1 <= K) {
-        nprop = 0;
-        for (int c0 = 0; c0 < nc; c0 += 64) {
-            const int c = c0 + lane;
-            const bool in = c < nc && c != self;
-            const uint64_t m = __ballot(in);
-            const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            if (in) props[at] = (uint32_t)c;
-            nprop += __popcll(m);
+    __syncthreads();
+
+    // ---- phase 2: one wave per reference block
+    for (int k = wave; k < kPropRefs; k += 4) {
+        const int bi = bi0 + k;
+        if (bi > last_bi) break;
+        const int b = bj * p.nx + bi;
+        const int i = grid_pos(bi, p.block_step, p.width - B);
+        int* cnt_out = a.count + b;
+        int* pos_out = a.pos + (size_t)b * G;
+        float* key_out = a.key + (size_t)b * G;
+        if (shortcut) {
+            if (lane == 0) {
+                *cnt_out = 1;
+                pos_out[0] = (j << 16) | i;
+                key_out[0] = 0.0f;
+            }
+            continue;
         }
-    } else {
-        const unsigned short* sc = reinterpret_cast<const unsigned short*>(a.scores) + (size_t)b * a.max_cand;
-        constexpr int NS = (kMaxWindow + 63) / 64;
-        uint32_t keys[NS];
+        const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
+        const int ncols = rr - l + 1, nc = ncols * nrows;
+        const int self = (j - t) * ncols + (i - l);
+        unsigned short* mine = sc_lds + k * rs;
+        // per-wave scratch aliases this block's scores once they are in registers
+        uint32_t* props = reinterpret_cast<uint32_t*>(mine);
+        uint32_t* sb = props + kMaxProposals;
+        int nprop = 0;
+        if (nc - 1 <= K) {
+            for (int c0 = 0; c0 < nc; c0 += 64) {
+                const int c = c0 + lane;
+                const bool in = c < nc && c != self;
+                const uint64_t m = __ballot(in);
+                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                wave_sync_lds();
+                if (in) props[at] = (uint32_t)c;
+                nprop += __popcll(m);
+            }
+        } else {
+            constexpr int NS = (kMaxWindow + 63) / 64;
+            uint32_t keys[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int c = lane + 64 * s;
-            keys[s] = (c < nc && c != self) ? (((uint32_t)sc[c] << 16) | (uint32_t)c) : 0xFFFFFFFFu;
+            for (int s = 0; s < NS; ++s) {
+                const int c = lane + 64 * s;
+                keys[s] = (c < nc && c != self) ? (((uint32_t)mine[c] << 16) | (uint32_t)c) : 0xFFFFFFFFu;
+            }
+            wave_sync_lds();
+            uint32_t lm = 0xFFFFFFFFu;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) lm = keys[s] < lm ? keys[s] : lm;
+            uint32_t t0 = 0;
+            for (int bit = 31; bit >= 0; --bit) {
+                const uint32_t trial = t0 | (1u << bit);
+                if (__popcll(__ballot(lm < trial)) < K) t0 = trial;
+            }
+            int M = 0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) M += __popcll(__ballot(keys[s] <= t0));
+            uint32_t tv = 0;
+            if (t0 != 0xFFFFFFFFu && M <= kSelBuf) {
+                int base = 0;
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const bool in = keys[s] <= t0;
+                    const uint64_t m = __ballot(in);
+                    const int at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (in) sb[at] = keys[s];
+                    base += __popcll(m);
+                }
+                wave_sync_lds();
+                constexpr int SR = kSelBuf / 64;
+                uint32_t ck[SR];
+#pragma unroll
+                for (int u = 0; u < SR; ++u) ck[u] = lane + 64 * u < M ? sb[lane + 64 * u] : 0xFFFFFFFFu;
+                for (int bit = 31; bit >= 0; --bit) {
+                    const uint32_t trial = tv | (1u << bit);
+                    int cnt = 0;
+#pragma unroll
+                    for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] < trial));
+                    if (cnt < K) tv = trial;
+                }
+            } else {
+                for (int bit = 31; bit >= 0; --bit) {
+                    const uint32_t trial = tv | (1u << bit);
+                    int cnt = 0;
+#pragma unroll
+                    for (int s = 0; s < NS; ++s) cnt += __popcll(__ballot(keys[s] < trial));
+                    if (cnt < K) tv = trial;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const bool in = keys[s] <= tv;
+                const uint64_t m = __ballot(in);
+                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                if (in) props[at] = keys[s] & 0xffffu;
+                nprop += __popcll(m);
+            }
         }
-        // radix descent for the K-th smallest key
-        uint32_t tv = 0;
-        for (int bit = 31; bit >= 0; --bit) {
-            const uint32_t trial = tv | (1u << bit);
-            int cnt = 0;
+        wave_sync_lds();
+        // ---- exact re-rank, one lane per proposal (nprop <= 64)
+        float rb[64];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) cnt += __popcll(__ballot(keys[s] < trial));
-            if (cnt < K) tv = trial;
-        }
-        nprop = 0;
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const bool in = keys[s] <= tv;
-            const uint64_t m = __ballot(in);
-            const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-            if (in) props[at] = keys[s] & 0xffffu;
-            nprop += __popcll(m);
-        }
-    }
-    wave_sync_lds();
-    // --- exact re-rank of the proposals (one lane per proposal)
-    float rb[64];
-#pragma unroll
-    for (int e = 0; e < 64; ++e) rb[e] = a.plane[(size_t)(j + e / 8) * p.pitch + i + e % 8];
-    uint32_t* ekey = ek[wave];
-    int nacc = 0;
-    for (int c0 = 0; c0 < nprop; c0 += 64) {
-        const int s = c0 + lane;
+        for (int e = 0; e < 64; ++e) rb[e] = a.plane[(size_t)(j + e / 8) * p.pitch + i + e % 8];
         uint32_t kv = 0xFFFFFFFFu;
-        int c = 0;
-        if (s < nprop) {
-            c = (int)props[s];
+        int c = 0x7FFFFFFF;
+        if (lane < nprop) {
+            c = (int)props[lane];
             const int cy = c / ncols, cx = c - cy * ncols;
-            const float dist = ssd_fixed<8>(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
+            const float dist = ssd8_rows(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
             if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
         }
-        const bool ok = kv != 0xFFFFFFFFu;
-        nacc += __popcll(__ballot(ok));
-        if (s < nprop) ekey[s] = kv;
-    }
-    wave_sync_lds();
-    const int kk = nacc < G - 1 ? nacc : G - 1;
-    if (nacc < G - 1 && nc - 1 > K) {
-        // not enough acceptable proposals: the exhaustive search decides
-        if (lane == 0) {
-            const int at = atomicAdd(a.nfallback, 1);
-            a.fallback[at] = b;
+        const int nacc = __popcll(__ballot(kv != 0xFFFFFFFFu));
+        const int kk = nacc < G - 1 ? nacc : G - 1;
+        if (nacc < G - 1 && nc - 1 > K) {
+            // not enough acceptable proposals: the exhaustive search decides
+            if (lane == 0) {
+                const int at = atomicAdd(a.nfallback, 1);
+                a.fallback[at] = b;
+            }
+            continue;
         }
-        return;
-    }
-    // rank the accepted proposals by (key, candidate index); keep the kk best
-    for (int s = lane; s < nprop; s += 64) {
-        const uint32_t kv = ekey[s];
-        if (kv == 0xFFFFFFFFu) continue;
-        const int c = (int)props[s];
+        // rank by (key, candidate index) against every other proposal
         int rank = 0;
         for (int o = 0; o < nprop; ++o) {
-            const uint32_t k2 = ekey[o];
-            const int c2 = (int)props[o];
+            const uint32_t k2 = __builtin_amdgcn_readlane(kv, o);
+            const int c2 = __builtin_amdgcn_readlane(c, o);
             rank += (k2 < kv) || (k2 == kv && c2 < c);
         }
-        if (rank < kk) {
+        if (kv != 0xFFFFFFFFu && rank < kk) {
             const int cy = c / ncols;
             pos_out[1 + rank] = ((t + cy) << 16) | (l + c - cy * ncols);
             key_out[1 + rank] = __uint_as_float(kv);
         }
-    }
-    if (lane == 0) {
-        *cnt_out = kk + 1;
-        pos_out[0] = (j << 16) | i;
-        key_out[0] = 0.0f;
+        if (lane == 0) {
+            *cnt_out = kk + 1;
+            pos_out[0] = (j << 16) | i;
+            key_out[0] = 0.0f;
+        }
     }
 }
+
+size_t propose_lds_bytes(int max_cand) { return (size_t)kPropRefs * prop_stride(max_cand) * sizeof(unsigned short); }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -706,14 +740,10 @@ hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int 
     return hipGetLastError();
 }
 
-hipError_t launch_score(const ScoreArgs& a, hipStream_t st) {
-    const int tiles = ((a.p.ny + kTileRY - 1) / kTileRY) * ((a.p.nx + kTileRX - 1) / kTileRX);
-    hipLaunchKernelGGL(k_score, dim3(tiles), dim3(256), score_lds_bytes(a.p), st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_select(const SelectArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_select, dim3((a.nb + 3) / 4), dim3(256), 0, st, a);
+hipError_t launch_propose(const ProposeArgs& a, hipStream_t st) {
+    const int groups = a.p.ny * ((a.p.nx + kPropRefs - 1) / kPropRefs);
+    if (groups == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_propose, dim3(groups), dim3(256), propose_lds_bytes(a.max_cand), st, a);
     return hipGetLastError();
 }
 

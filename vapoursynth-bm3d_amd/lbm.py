@@ -88,12 +88,14 @@ class Model:
 
     # plane: torch float32 CUDA tensor (H, W), contiguous; pairs: torch int32 CUDA (n, 4)
     def train_step(self, plane, pairs, lr: float) -> float:
+        bm3d._sync_producer(plane.device)
         loss = C.c_float()
         self._check(self.L.bm3d_lbm_train_step(self.h, plane.data_ptr(), plane.stride(0), pairs.data_ptr(),
                                                pairs.shape[0], lr, C.byref(loss)))
         return loss.value
 
     def gradients(self, plane, pairs, target=None):
+        bm3d._sync_producer(plane.device)
         g = np.zeros(N_PARAMS, np.float32)
         loss = C.c_float()
         self._check(self.L.bm3d_lbm_gradients(self.h, plane.data_ptr(), plane.stride(0), pairs.data_ptr(),
@@ -102,6 +104,7 @@ class Model:
         return g, loss.value
 
     def forward(self, plane, pairs) -> np.ndarray:
+        bm3d._sync_producer(plane.device)
         e = np.zeros((2 * pairs.shape[0], K_EMB), np.float32)
         self._check(self.L.bm3d_lbm_forward(self.h, plane.data_ptr(), plane.stride(0), pairs.data_ptr(),
                                             pairs.shape[0], _fp(e)))
@@ -124,6 +127,7 @@ def match_plane(flt, model: Optional[Model], plane, topk: int = 32):
     key = torch.zeros(nb * G, dtype=torch.float32, device=dev)
     fb = C.c_int()
     ms = C.c_float()
+    bm3d._sync_producer(dev)
     rc = flt.L.bm3d_lbm_match_plane(flt.h, model.h if model is not None else None, topk, plane.data_ptr(),
                                     plane.stride(0), W, H, count.data_ptr(), pos.data_ptr(), key.data_ptr(),
                                     C.byref(fb), C.byref(ms))
