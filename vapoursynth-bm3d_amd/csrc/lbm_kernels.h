@@ -73,7 +73,6 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
                            hipStream_t st);
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st);
-size_t propose_lds_bytes(int max_cand);
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
 
 }  // namespace lbm

@@ -364,14 +364,25 @@ __global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ 
     Frags f;
     load_frags(n, f);
     const int r = lane & 15, q = lane >> 4;
+    s4 nf[4];
+    {
+        float raw[4];
+        const int x = xs + r < ew ? xs + r : ew - 1;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) nf[s] = patch_frag(plane, pitch, y, x, s, raw);
+    }
     for (int g = 0; g < kEmbedGroups; ++g) {
         const int x0 = xs + 16 * g;
         if (x0 >= ew) break;
-        const int x = x0 + r < ew ? x0 + r : ew - 1;
         s4 xf[4];
-        float raw[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) xf[s] = patch_frag(plane, pitch, y, x, s, raw);
+        for (int s = 0; s < 4; ++s) xf[s] = nf[s];
+        if (g + 1 < kEmbedGroups && x0 + 16 < ew) {  // next group's patches in flight
+            float raw[4];
+            const int xn = x0 + 16 + r < ew ? x0 + 16 + r : ew - 1;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) nf[s] = patch_frag(plane, pitch, y, xn, s, raw);
+        }
         float h[8][4], e[2][4];
         forward16(f, xf, h, e);
         float nrm = 0.0f;
@@ -404,37 +415,35 @@ __global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ 
 // inference: proposals, exact re-rank and fallback list
 // ---------------------------------------------------------------------------
 //
-// One workgroup (four waves) per row of 16 reference blocks.
+// One workgroup (four waves) per tile of kPropRows x kPropCols reference
+// blocks.  Embedding distance: score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c
+// (fp32; fp16 in keys).  The dot products of the 16 blocks of a tile row with
+// 16 positions of a union row are one 16x16x32 product (two
+// v_mfma_f32_16x16x16_f16); a candidate embedding is loaded once per tile and
+// serves every tile row whose windows hold that union row.  Tiles are dealt
+// to the XCDs in column-major runs, so vertically adjacent tiles, whose
+// unions overlap most, share an L2.
 //
-// Scores: score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c (clamped at 0) of every
-// candidate c of each block's search window, as fp16 in LDS at
-// [block][cy * ncols + cx] (window row-major).  The dot products of the 16
-// blocks with 16 consecutive positions of a union row are one 16x16x32
-// product (two v_mfma_f32_16x16x16_f16), so every candidate embedding is read
-// once per workgroup; the waves take interleaved union rows.
+// Pass 1 (per tile row) scores the quarter lattice of each window (every
+// other row, even columns) and takes each block's (K/2)-th best sub-score as
+// its threshold tau (about 2K full-window scores fall below it).  Pass 2
+// scores every candidate and appends the ones with score <= tau to the
+// block's list in LDS as keys (fp16 score bits << 16 | window index): unique,
+// so ordering keys orders by score with ties to the earlier candidate.
 //
-// Proposals (one wave per block): keys (fp16 score bits << 16 | candidate
-// index) are unique, so the K smallest are the K best scores with ties going
-// to the earlier candidate.  The K-th smallest lane minimum t0 bounds the
-// K-th smallest key (K lane minima are <= t0); the keys <= t0 (usually a few
-// K) are compacted and the exact K-th smallest is found among them.
-//
-// Re-rank: the K proposals are scored exactly like the plain matcher
-// (SSE2 lane-order SSD, dist <= thSSE && dist != 0, key = float(double(dist)
-// * distMul)) and the G-1 best by (key, candidate index) follow the self
-// entry.  A block whose window holds no more than K candidates is searched
-// exhaustively here; a block with fewer than G-1 accepted proposals is
-// appended to the fallback list.
-constexpr int kPropRefs = 16;
-constexpr int kSelBuf = 256;
-
-// LDS stride (fp16 elements) of one block's scores: room for the whole window
-// and for the wave's selection scratch (kMaxProposals + kSelBuf words) that
-// later aliases it; even, so the scratch is 4-byte aligned.
-__host__ __device__ __forceinline__ int prop_stride(int max_cand) {
-    const int need = 2 * (kMaxProposals + kSelBuf);
-    return ((max_cand > need ? max_cand : need) + 1) & ~1;
-}
+// Per block (one wave): the K smallest keys of the list are the proposals
+// (the list must hold at least K and at most kListCap keys, else the block
+// falls back to the exhaustive search).  The proposals are re-ranked exactly
+// like the plain matcher (SSE2 lane-order SSD, dist <= thSSE && dist != 0,
+// key = float(double(dist) * distMul)); the G-1 best by (key, window index)
+// follow the self entry.  A block with fewer than G-1 accepted proposals also
+// falls back.  A window of at most K candidates is searched exhaustively here.
+constexpr int kPropCols = 16;
+constexpr int kPropRows = 4;
+constexpr int kPropBlocks = kPropRows * kPropCols;
+constexpr int kListCap = 128;
+constexpr int kSubCap = 25 * 25;  // quarter lattice of the largest window
+constexpr int kBitWords = (kMaxWindow + 31) / 32;
 
 __device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
@@ -465,91 +474,230 @@ __device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restr
     return 0.0f + (((s0 + s1) + s2) + s3);
 }
 
+// k-th smallest (1-based) of the wave's values v[0..N) (0xFFFFFFFF = absent).
+template <int N>
+__device__ __forceinline__ uint32_t wave_kth(const uint32_t (&v)[N], int k) {
+    uint32_t tv = 0;
+    for (int bit = 31; bit >= 0; --bit) {
+        const uint32_t trial = tv | (1u << bit);
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < N; ++u) cnt += __popcll(__ballot(v[u] < trial));
+        if (cnt < k) tv = trial;
+    }
+    return tv;
+}
+
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
-    extern __shared__ unsigned short sc_lds[];  // [kPropRefs][prop_stride] fp16 scores
+    __shared__ unsigned short sub[kPropCols][kSubCap];  // pass-1 scores of one tile row
+    __shared__ uint32_t bits[kPropBlocks][kBitWords];   // pass 2: candidates under tau
+    __shared__ unsigned short tau[kPropBlocks];
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
     const int r = lane & 15, q = lane >> 4;
-    const int col_tiles = (p.nx + kPropRefs - 1) / kPropRefs;
-    const int bj = blockIdx.x / col_tiles;
-    const int bi0 = (blockIdx.x - bj * col_tiles) * kPropRefs;
+    // tile of the workgroup (XCD-contiguous, column-major)
+    const int row_tiles = (p.ny + kPropRows - 1) / kPropRows;
+    const int col_tiles = (p.nx + kPropCols - 1) / kPropCols;
+    const int ntiles = row_tiles * col_tiles;
+    const int per_xcd = (ntiles + 7) >> 3;
+    const int lin = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (lin >= ntiles) return;
+    const int tcol = lin / row_tiles;
+    const int bj0 = (lin - tcol * row_tiles) * kPropRows;
+    const int bi0 = tcol * kPropCols;
     const int ew = p.width - 7;
     const int B = 8;
     const int G = p.group;
     const int K = a.topk;
-    const int last_bi = bi0 + kPropRefs - 1 < p.nx - 1 ? bi0 + kPropRefs - 1 : p.nx - 1;
-    const int j = grid_pos(bj, p.block_step, p.height - B);
-    const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
-    const int nrows = bt - t + 1;
+    const int last_bi = bi0 + kPropCols - 1 < p.nx - 1 ? bi0 + kPropCols - 1 : p.nx - 1;
     const bool shortcut = G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0);
-    const int rs = prop_stride(a.max_cand);
+    const int sub_rank = K / 2 > 1 ? K / 2 : 1;
+    // tile rows
+    int jr[kPropRows], tr[kPropRows], br[kPropRows];
+    bool rok[kPropRows];
+#pragma unroll
+    for (int m = 0; m < kPropRows; ++m) {
+        rok[m] = bj0 + m < p.ny;
+        jr[m] = grid_pos(rok[m] ? bj0 + m : p.ny - 1, p.block_step, p.height - B);
+        tr[m] = search_lo(jr[m], p.range, 1);
+        br[m] = search_hi(jr[m], p.height - B, p.range, 1);
+    }
 
-    // ---- phase 1: window scores into LDS
     if (!shortcut) {
+        for (int e = threadIdx.x; e < kPropBlocks * kBitWords; e += 256) (&bits[0][0])[e] = 0u;
+        // columns: A row r <-> block bi0 + r; D row 4q + ii <-> block bi0 + 4q + ii
         const int ia = grid_pos(bi0 + r <= last_bi ? bi0 + r : last_bi, p.block_step, p.width - B);
-        int lcol[4], ncol[4];
-        bool cok[4];
-        float nref[4];
+        int lcol[4], ncol[4], icol[4];
+        uint32_t ncolv[4];
 #pragma unroll
         for (int ii = 0; ii < 4; ++ii) {
             const int bi = bi0 + 4 * q + ii;
-            cok[ii] = bi <= last_bi;
-            const int i = grid_pos(cok[ii] ? bi : last_bi, p.block_step, p.width - B);
-            lcol[ii] = search_lo(i, p.range, 1);
-            ncol[ii] = search_hi(i, p.width - B, p.range, 1) - lcol[ii] + 1;
-            nref[ii] = a.norm[(size_t)j * ew + i];
+            const bool ok = bi <= last_bi;
+            icol[ii] = grid_pos(ok ? bi : last_bi, p.block_step, p.width - B);
+            lcol[ii] = search_lo(icol[ii], p.range, 1);
+            ncol[ii] = search_hi(icol[ii], p.width - B, p.range, 1) - lcol[ii] + 1;
+            ncolv[ii] = ok ? (uint32_t)ncol[ii] : 0u;  // 0: no column passes the window test
         }
         const int u_l = search_lo(grid_pos(bi0, p.block_step, p.width - B), p.range, 1);
         const int u_r = search_hi(grid_pos(last_bi, p.block_step, p.width - B), p.width - B, p.range, 1);
-        h4 af0, af1;
-        {
-            const _Float16* er = a.emb + ((size_t)j * ew + ia) * kEmb;
-            __builtin_memcpy(&af0, er + 4 * q, sizeof(af0));
-            __builtin_memcpy(&af1, er + 16 + 4 * q, sizeof(af1));
+        h4 af[kPropRows][2];
+        float nref[kPropRows][4];
+#pragma unroll
+        for (int m = 0; m < kPropRows; ++m) {
+            const _Float16* er = a.emb + ((size_t)jr[m] * ew + ia) * kEmb;
+            __builtin_memcpy(&af[m][0], er + 4 * q, sizeof(h4));
+            __builtin_memcpy(&af[m][1], er + 16 + 4 * q, sizeof(h4));
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) nref[m][ii] = a.norm[(size_t)jr[m] * ew + icol[ii]];
         }
-        const int ncb = (u_r - u_l + 16) >> 4;
-        const int ntile = nrows * ncb;
-        // tiles (union row, column block) dealt to the waves; B fragments one tile ahead
-        int tl = wave;
-        h4 b0 = {}, b1 = {};
-        float nq = 0.0f;
-        auto load_tile = [&](int tt, h4& x0f, h4& x1f, float& nqf) {
-            const int yy = t + tt / ncb;
-            const int xx = u_l + (tt - (tt / ncb) * ncb) * 16 + r;
-            const int xl = xx <= u_r ? xx : u_r;
-            const _Float16* ec = a.emb + ((size_t)yy * ew + xl) * kEmb;
+        auto load_pos = [&](int yy, int xx, h4& x0f, h4& x1f, float& nqf) {
+            const _Float16* ec = a.emb + ((size_t)yy * ew + xx) * kEmb;
             __builtin_memcpy(&x0f, ec + 4 * q, sizeof(x0f));
             __builtin_memcpy(&x1f, ec + 16 + 4 * q, sizeof(x1f));
-            nqf = a.norm[(size_t)yy * ew + xl];
+            nqf = a.norm[(size_t)yy * ew + xx];
         };
-        if (tl < ntile) load_tile(tl, b0, b1, nq);
-        for (; tl < ntile; tl += 4) {
-            h4 c0 = b0, c1 = b1;
-            const float cq = nq;
-            if (tl + 4 < ntile) load_tile(tl + 4, b0, b1, nq);
-            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc = mfma_h(af0, c0, acc);
-            acc = mfma_h(af1, c1, acc);
-            const int cy = tl / ncb;
-            const int x = u_l + (tl - cy * ncb) * 16 + r;
+
+        // ---- pass 1, one tile row at a time: quarter lattice, thresholds
+        const int u_le = u_l & ~1;
+        const int ncb1 = ((u_r - u_le) / 2 + 16) >> 4;  // blocks of 16 even columns
+        const int xlast = (u_r & ~1) >= u_l ? (u_r & ~1) : u_l;
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) {
-                const int cx = x - lcol[ii];
-                if (!cok[ii] || x > u_r || cx < 0 || cx >= ncol[ii]) continue;
-                float d = nref[ii] + cq - 2.0f * acc[ii];
-                d = d > 0.0f ? d : 0.0f;
-                sc_lds[(4 * q + ii) * rs + cy * ncol[ii] + cx] = __builtin_bit_cast(unsigned short, (_Float16)d);
+        for (int m = 0; m < kPropRows; ++m) {
+            if (!rok[m]) break;
+            for (int e = threadIdx.x; e < kPropCols * kSubCap; e += 256) (&sub[0][0])[e] = 0xFFFFu;
+            __syncthreads();
+            const int nrows = br[m] - tr[m] + 1;
+            const int nr2 = (nrows + 1) >> 1;
+            const int ntile = nr2 * ncb1;
+            for (int tl = wave; tl < ntile; tl += 4) {
+                const int mr = tl / ncb1;
+                const int x = u_le + 2 * ((tl - mr * ncb1) * 16 + r);
+                h4 b0, b1;
+                float nq;
+                load_pos(tr[m] + 2 * mr, x <= u_r ? x : xlast, b0, b1, nq);
+                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = mfma_h(af[m][0], b0, acc);
+                acc = mfma_h(af[m][1], b1, acc);
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) {
+                    const int cx = x - lcol[ii];
+                    if ((uint32_t)cx >= ncolv[ii]) continue;
+                    float d = __builtin_fmaf(-2.0f, acc[ii], nref[m][ii] + nq);
+                    d = d > 0.0f ? d : 0.0f;
+                    const int nc2 = (ncol[ii] + 1) >> 1;
+                    sub[4 * q + ii][mr * nc2 + (cx >> 1)] = __builtin_bit_cast(unsigned short, (_Float16)d);
+                }
+            }
+            __syncthreads();
+            for (int k = wave; k < kPropCols; k += 4) {
+                const int bi = bi0 + k;
+                if (bi > last_bi) break;
+                const int i = grid_pos(bi, p.block_step, p.width - B);
+                const int l = search_lo(i, p.range, 1);
+                const int ncols = search_hi(i, p.width - B, p.range, 1) - l + 1;
+                const int nsub = nr2 * ((ncols + 1) >> 1);
+                // tau: the sub_rank-th smallest of the lanes' minimum sub-scores
+                // (a lane holds ~nsub/64 of them), close to the sub_rank-th
+                // smallest sub-score itself at a fraction of the cost
+                constexpr int NS2 = (kSubCap + 63) / 64;
+                uint32_t lm[1] = {0xFFFFFFFFu};
+#pragma unroll
+                for (int u = 0; u < NS2; ++u) {
+                    const int e = lane + 64 * u;
+                    const uint32_t v = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFFFFFu;
+                    lm[0] = v < lm[0] ? v : lm[0];
+                }
+                const int present = __popcll(__ballot(lm[0] != 0xFFFFFFFFu));
+                const uint32_t tv = wave_kth<1>(lm, sub_rank < present ? sub_rank : present);
+                if (lane == 0) tau[m * kPropCols + k] = (unsigned short)(tv > 0xFFFFu ? 0xFFFFu : tv);
+            }
+            __syncthreads();
+        }
+
+        // ---- pass 2: every candidate of the union of the tile's windows
+        float tauf[kPropRows][4];
+#pragma unroll
+        for (int m = 0; m < kPropRows; ++m)
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii)
+                tauf[m][ii] = rok[m] ? (float)__builtin_bit_cast(_Float16, tau[m * kPropCols + 4 * q + ii]) : -1.0f;
+        int u_t = tr[0], u_b = br[0];
+#pragma unroll
+        for (int m = 1; m < kPropRows; ++m)
+            if (rok[m]) u_b = br[m];
+        const int ncb = (u_r - u_l + 16) >> 4;
+        const int ntile = (u_b - u_t + 1) * ncb;
+        constexpr int kPf = 4;  // tiles per group; the next group's loads are in flight
+        h4 g0[kPf], g1[kPf];
+        float gq[kPf];
+        auto load_group = [&](int tl0) {
+#pragma unroll
+            for (int u = 0; u < kPf; ++u) {
+                const int tt = tl0 + 4 * u < ntile ? tl0 + 4 * u : ntile - 1;
+                const int cy = tt / ncb;
+                const int x = u_l + (tt - cy * ncb) * 16 + r;
+                load_pos(u_t + cy, x <= u_r ? x : u_r, g0[u], g1[u], gq[u]);
+            }
+        };
+        if (wave < ntile) load_group(wave);
+        for (int tl0 = wave; tl0 < ntile; tl0 += 4 * kPf) {
+            h4 c0[kPf], c1[kPf];
+            float cq[kPf];
+#pragma unroll
+            for (int u = 0; u < kPf; ++u) {
+                c0[u] = g0[u];
+                c1[u] = g1[u];
+                cq[u] = gq[u];
+            }
+            if (tl0 + 4 * kPf < ntile) load_group(tl0 + 4 * kPf);
+#pragma unroll
+            for (int u = 0; u < kPf; ++u) {
+                const int tl = tl0 + 4 * u;
+                if (tl >= ntile) break;
+                const int cyu = tl / ncb;
+                const int y = u_t + cyu;
+                const int x = u_l + (tl - cyu * ncb) * 16 + r;
+#pragma unroll
+                for (int m = 0; m < kPropRows; ++m) {
+                    if (!rok[m] || y < tr[m] || y > br[m]) continue;
+                    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    acc = mfma_h(af[m][0], c0[u], acc);
+                    acc = mfma_h(af[m][1], c1[u], acc);
+                    uint32_t pass = 0;
+                    float dv[4];
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) {
+                        dv[ii] = __builtin_fmaf(-2.0f, acc[ii], nref[m][ii] + cq[u]);
+                        const bool ok = (uint32_t)(x - lcol[ii]) < ncolv[ii] && dv[ii] <= tauf[m][ii] &&
+                                        !(y == jr[m] && x == icol[ii]);
+                        pass |= ok ? (1u << ii) : 0u;
+                    }
+                    if (__ballot(pass != 0) == 0) continue;
+                    const int cy = y - tr[m];
+#pragma unroll
+                    for (int ii = 0; ii < 4; ++ii) {
+                        if (!(pass & (1u << ii))) continue;
+                        const int c = cy * ncol[ii] + (x - lcol[ii]);
+                        // no return value: the or is not waited for
+                        __hip_atomic_fetch_or(&bits[m * kPropCols + 4 * q + ii][c >> 5], 1u << (c & 31),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
             }
         }
     }
     __syncthreads();
 
-    // ---- phase 2: one wave per reference block
-    for (int k = wave; k < kPropRefs; k += 4) {
+    // ---- per block: proposals, exact re-rank, output
+    uint32_t* props_all = reinterpret_cast<uint32_t*>(&sub[0][0]);  // pass-1 scores are dead now
+    for (int kb = wave; kb < kPropBlocks; kb += 4) {
+        const int m = kb / kPropCols, k = kb - m * kPropCols;
         const int bi = bi0 + k;
-        if (bi > last_bi) break;
-        const int b = bj * p.nx + bi;
+        if (!rok[m] || bi > last_bi) continue;
+        const int j = jr[m], t = tr[m];
+        const int nrows = br[m] - t + 1;
+        const int b = (bj0 + m) * p.nx + bi;
         const int i = grid_pos(bi, p.block_step, p.width - B);
         int* cnt_out = a.count + b;
         int* pos_out = a.pos + (size_t)b * G;
@@ -565,81 +713,92 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
         const int ncols = rr - l + 1, nc = ncols * nrows;
         const int self = (j - t) * ncols + (i - l);
-        unsigned short* mine = sc_lds + k * rs;
-        // per-wave scratch aliases this block's scores once they are in registers
-        uint32_t* props = reinterpret_cast<uint32_t*>(mine);
-        uint32_t* sb = props + kMaxProposals;
+        uint32_t* props = props_all + wave * kMaxProposals;
         int nprop = 0;
+        bool fallback = false;
         if (nc - 1 <= K) {
             for (int c0 = 0; c0 < nc; c0 += 64) {
                 const int c = c0 + lane;
                 const bool in = c < nc && c != self;
-                const uint64_t m = __ballot(in);
-                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                wave_sync_lds();
+                const uint64_t mk = __ballot(in);
+                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
                 if (in) props[at] = (uint32_t)c;
-                nprop += __popcll(m);
+                nprop += __popcll(mk);
             }
         } else {
-            constexpr int NS = (kMaxWindow + 63) / 64;
-            uint32_t keys[NS];
+            // candidates under tau: the set bits of the block's bitmap
+            const int nw = (nc + 31) >> 5;
+            uint32_t w0 = lane < nw ? bits[kb][lane] : 0u;
+            uint32_t w1 = lane + 64 < nw ? bits[kb][lane + 64] : 0u;
+            const int c0n = __popc(w0), c1n = __popc(w1);
+            // exclusive prefix of the per-lane counts (word order = candidate order)
+            int incl = c0n;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int c = lane + 64 * s;
-                keys[s] = (c < nc && c != self) ? (((uint32_t)mine[c] << 16) | (uint32_t)c) : 0xFFFFFFFFu;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
             }
-            wave_sync_lds();
-            uint32_t lm = 0xFFFFFFFFu;
+            const int tot0 = __shfl(incl, 63);
+            int incl1 = c1n;
 #pragma unroll
-            for (int s = 0; s < NS; ++s) lm = keys[s] < lm ? keys[s] : lm;
-            uint32_t t0 = 0;
-            for (int bit = 31; bit >= 0; --bit) {
-                const uint32_t trial = t0 | (1u << bit);
-                if (__popcll(__ballot(lm < trial)) < K) t0 = trial;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl1, o);
+                if (lane >= o) incl1 += v;
             }
-            int M = 0;
-#pragma unroll
-            for (int s = 0; s < NS; ++s) M += __popcll(__ballot(keys[s] <= t0));
-            uint32_t tv = 0;
-            if (t0 != 0xFFFFFFFFu && M <= kSelBuf) {
-                int base = 0;
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    const bool in = keys[s] <= t0;
-                    const uint64_t m = __ballot(in);
-                    const int at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                    if (in) sb[at] = keys[s];
-                    base += __popcll(m);
-                }
+            const int n = tot0 + __shfl(incl1, 63);
+            if (n < K || n > kListCap) {
+                fallback = true;
+            } else {
+                uint32_t* cand = props_all + 4 * kMaxProposals + wave * kListCap;
+                int at = incl - c0n;
+                for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
+                at = tot0 + incl1 - c1n;
+                for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
                 wave_sync_lds();
-                constexpr int SR = kSelBuf / 64;
+                // scores of the listed candidates (fp16 bits as in pass 2) -> keys
+                const _Float16* er = a.emb + ((size_t)j * ew + i) * kEmb;
+                uint32_t ew32[16];
+                __builtin_memcpy(ew32, er, sizeof(ew32));
+                const float nr = a.norm[(size_t)j * ew + i];
+                constexpr int SR = kListCap / 64;
                 uint32_t ck[SR];
 #pragma unroll
-                for (int u = 0; u < SR; ++u) ck[u] = lane + 64 * u < M ? sb[lane + 64 * u] : 0xFFFFFFFFu;
-                for (int bit = 31; bit >= 0; --bit) {
-                    const uint32_t trial = tv | (1u << bit);
-                    int cnt = 0;
+                for (int u = 0; u < SR; ++u) {
+                    const int e = lane + 64 * u;
+                    ck[u] = 0xFFFFFFFFu;
+                    if (e < n) {
+                        const int c = (int)cand[e];
+                        const int cy = c / ncols, cx = c - cy * ncols;
+                        const size_t pq = (size_t)(t + cy) * ew + l + cx;
+                        uint32_t eq[16];
+                        __builtin_memcpy(eq, a.emb + pq * kEmb, sizeof(eq));
+                        float dot = 0.0f;
 #pragma unroll
-                    for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] < trial));
-                    if (cnt < K) tv = trial;
+                        for (int w = 0; w < 16; ++w)
+                            dot = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, ew32[w]), __builtin_bit_cast(h2, eq[w]),
+                                                         dot, false);
+                        float d = __builtin_fmaf(-2.0f, dot, nr + a.norm[pq]);
+                        d = d > 0.0f ? d : 0.0f;
+                        ck[u] = ((uint32_t)__builtin_bit_cast(unsigned short, (_Float16)d) << 16) | (uint32_t)c;
+                    }
                 }
-            } else {
-                for (int bit = 31; bit >= 0; --bit) {
-                    const uint32_t trial = tv | (1u << bit);
-                    int cnt = 0;
+                const uint32_t tv = wave_kth<SR>(ck, K);
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) cnt += __popcll(__ballot(keys[s] < trial));
-                    if (cnt < K) tv = trial;
+                for (int u = 0; u < SR; ++u) {
+                    const bool in = ck[u] <= tv;
+                    const uint64_t mk = __ballot(in);
+                    const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
+                    if (in) props[at] = ck[u] & 0xffffu;
+                    nprop += __popcll(mk);
                 }
             }
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const bool in = keys[s] <= tv;
-                const uint64_t m = __ballot(in);
-                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                if (in) props[at] = keys[s] & 0xffffu;
-                nprop += __popcll(m);
+        }
+        if (fallback) {
+            if (lane == 0) {
+                const int at = atomicAdd(a.nfallback, 1);
+                a.fallback[at] = b;
             }
+            continue;
         }
         wave_sync_lds();
         // ---- exact re-rank, one lane per proposal (nprop <= 64)
@@ -654,6 +813,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             const float dist = ssd8_rows(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
             if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
         }
+        wave_sync_lds();  // props is rewritten by this wave's next block
         const int nacc = __popcll(__ballot(kv != 0xFFFFFFFFu));
         const int kk = nacc < G - 1 ? nacc : G - 1;
         if (nacc < G - 1 && nc - 1 > K) {
@@ -664,7 +824,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             }
             continue;
         }
-        // rank by (key, candidate index) against every other proposal
+        // rank by (key, window index) against every other proposal
         int rank = 0;
         for (int o = 0; o < nprop; ++o) {
             const uint32_t k2 = __builtin_amdgcn_readlane(kv, o);
@@ -683,8 +843,6 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         }
     }
 }
-
-size_t propose_lds_bytes(int max_cand) { return (size_t)kPropRefs * prop_stride(max_cand) * sizeof(unsigned short); }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -741,9 +899,9 @@ hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int 
 }
 
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st) {
-    const int groups = a.p.ny * ((a.p.nx + kPropRefs - 1) / kPropRefs);
-    if (groups == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_propose, dim3(groups), dim3(256), propose_lds_bytes(a.max_cand), st, a);
+    const int tiles = ((a.p.ny + kPropRows - 1) / kPropRows) * ((a.p.nx + kPropCols - 1) / kPropCols);
+    if (tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_propose, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
