@@ -11,6 +11,8 @@ kernel's algorithmic FLOP rate (per-launch FLOPs / average launch time from
 HIP events recorded on the library's stream) against the MI355X peak.
 cpu_baseline = the CPU oracle (a C restatement of the reference CPU path,
 one frame per thread like VapourSynth's fmParallel) on a bounded sample.
+learned_bm = the secondary metric: learned-proposal vs exhaustive block
+matching on a 4K Gray frame (speedup, recall, PSNR loss).
 """
 from __future__ import annotations
 
@@ -64,6 +66,50 @@ def measured_traffic(timer):
     return None
 
 
+LEARNED_KW = dict(profile="fast", sigma=25.0, bm_range=24, group_size=16)
+LEARNED_TOPK = 32
+
+
+def learned_leg(reps=3):
+    """Secondary metric: block matching of Basic(profile='fast', sigma=25,
+    bm_range=24, group_size=16) on one seeded 3840x2160 Gray float frame,
+    exhaustive vs the learned proposal (top-32 re-ranked by exact SSD, trained
+    weights of lbm_train.py).  Matching time per frame from the library's
+    event timers; recall of the exhaustive lists; PSNR of both Basic outputs
+    against the clean image."""
+    import bm3d
+    import lbm
+    import synth
+    weights = os.path.join(ROOT, "vapoursynth-bm3d_amd", "lbm_weights.npz")
+    if not os.path.exists(weights):
+        return {"skipped": "no trained weights (run vapoursynth-bm3d_amd/lbm_train.py)"}
+    noisy, clean = synth.gray_float(2160, 3840, 25.0, seed=5999)
+    clip = bm3d.Clip.from_arrays(bm3d.GRAYS, [noisy[None]])
+    model = lbm.Model.load(weights)
+    res = {}
+    for name, m in (("exhaustive", None), ("learned", model)):
+        node = bm3d.Basic(clip, **LEARNED_KW)
+        if m is not None:
+            node.filter.set_matcher(m, LEARNED_TOPK)
+        node.get_frame(0)  # warm-up
+        node.filter.profile(True)
+        for _ in range(reps):
+            node.clear_cache()
+            out = np.asarray(node.get_frame(0).planes[0])
+        prof = node.filter.profile_read()
+        ms, n = prof["k_match(basic)"]
+        cnt, pos, _ = node.filter.match_lists()
+        res[name] = dict(ms=ms / n, out=out, cnt=cnt, pos=pos, fallback=node.filter.matcher_fallbacks())
+    e, l = res["exhaustive"], res["learned"]
+    p_e, p_l = lbm.psnr(e["out"], clean), lbm.psnr(l["out"], clean)
+    return {"workload": "Basic(fast, sigma=25, bm_range=24, group_size=16), 1 frame 3840x2160 Gray float, top-32",
+            "exhaustive_match_ms": round(e["ms"], 4), "learned_match_ms": round(l["ms"], 4),
+            "speedup": round(e["ms"] / l["ms"], 3), "recall": round(lbm.recall(e["cnt"], e["pos"], l["cnt"], l["pos"]), 4),
+            "psnr_exhaustive_db": round(p_e, 4), "psnr_learned_db": round(p_l, 4),
+            "psnr_loss_db": round(p_e - p_l, 4), "psnr_learned_vs_exhaustive_db": round(lbm.psnr(l["out"], e["out"]), 3),
+            "fallback_blocks": l["fallback"], "ref_blocks": int(len(e["cnt"]))}
+
+
 def cpu_baseline(frames, rows, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -88,6 +134,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=128)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned block-matching leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (three times) and exit; "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters")
@@ -194,6 +241,9 @@ def main():
                      "flops_per_launch": flops_launch},
         "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kern.items()},
     }
+    if not args.no_learned:
+        print("learned block-matching leg (4K)", file=sys.stderr, flush=True)
+        result["learned_bm"] = learned_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames, args.cpu_rows, min(FRAMES, os.cpu_count() or 1))
     print(json.dumps(result), flush=True)
