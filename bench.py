@@ -12,7 +12,10 @@ HIP events recorded on the library's stream) against the MI355X peak.
 cpu_baseline = the CPU oracle (a C restatement of the reference CPU path,
 one frame per thread like VapourSynth's fmParallel) on a bounded sample.
 learned_bm = the secondary metric: learned-proposal vs exhaustive block
-matching on a 4K Gray frame (speedup, recall, PSNR loss).
+matching on a 4K Gray frame (speedup, recall, PSNR loss).  batched = Basic ->
+Final over 64 Gray float frames of uneven sizes with per-frame sigma, one
+persistent launch per stage, HIP-graph replay, against the single-frame
+filters.
 """
 from __future__ import annotations
 
@@ -110,6 +113,71 @@ def learned_leg(reps=3):
             "fallback_blocks": l["fallback"], "ref_blocks": int(len(e["cnt"]))}
 
 
+BATCH_FRAMES = 64
+
+
+def batch_workload(n=BATCH_FRAMES, seed=64):
+    """64 Gray float frames of uneven sizes between 256x256 and 2048x1152
+    (the two extremes included) with per-frame sigma in [5, 30]."""
+    rng = np.random.RandomState(seed)
+    hs = rng.randint(256, 1153, n)
+    ws = rng.randint(256, 2049, n)
+    hs[0], ws[0], hs[1], ws[1] = 256, 256, 1152, 2048
+    sig = np.round(rng.uniform(5.0, 30.0, n), 2)
+    return [(int(h), int(w)) for h, w in zip(hs, ws)], [float(s) for s in sig]
+
+
+def batched_leg(reps=3, tune=True):
+    """Batched Basic(np) -> Final(np) over the 64-frame uneven workload: one
+    persistent launch per stage for all frames, the chain captured in a HIP
+    graph after autotuning the launch shape.  Compared with the same frames
+    filtered one at a time through the single-frame filters (steady state,
+    each frame's filters created and warmed up before its timed call), and
+    checked bit for bit against them."""
+    import torch
+    import bm3d
+    import synth
+    sizes, sigmas = batch_workload()
+    src, dst = [], []
+    for k, ((h, w), s) in enumerate(zip(sizes, sigmas)):
+        src.append(torch.from_numpy(synth.gray_float(h, w, s, seed=6400 + k)[0]).cuda())
+        dst.append(torch.empty((h, w), dtype=torch.float32, device="cuda"))
+    b = bm3d.Batch(sizes, sigmas, profile="np")
+    info = b.info()
+    b.run(src, dst)  # warm-up
+    rep = b.tune(src, dst, reps=1) if tune else ""
+    eager = [b.run(src, dst) for _ in range(reps)]
+    b.run(src, dst, graph=True)  # capture
+    graph = [b.run(src, dst, graph=True) for _ in range(reps)]
+    torch.cuda.synchronize()
+    # single-frame filters, frame by frame
+    single_ms, exact = 0.0, True
+    for k, ((h, w), s) in enumerate(zip(sizes, sigmas)):
+        clip = bm3d.Clip.from_arrays(bm3d.GRAYS, [src[k][None]])
+        basic = bm3d.Basic(clip, profile="np", sigma=s)
+        final = bm3d.Final(clip, basic, profile="np", sigma=s)
+        final.get_frame(0)
+        basic.clear_cache()
+        final.clear_cache()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = final.get_frame(0).planes[0]
+        torch.cuda.synchronize()
+        single_ms += (time.perf_counter() - t0) * 1e3
+        exact = exact and bool(torch.equal(out.view(torch.int32), dst[k].view(torch.int32)))
+        del clip, basic, final, out
+    ms = min(graph)
+    px = info["pixels"]
+    return {"workload": f"Basic(np)->Final(np), {len(sizes)} Gray float frames 256x256..2048x1152, "
+                        f"sigma 5..30 per frame, inputs resident in HBM",
+            "frames": len(sizes), "pixels": px, "basic_blocks": info["basic_blocks"],
+            "final_blocks": info["final_blocks"],
+            "graph_ms": round(ms, 3), "eager_ms": round(min(eager), 3), "mpix_s": round(px / ms / 1e3, 3),
+            "single_frame_ms": round(single_ms, 3), "single_frame_mpix_s": round(px / single_ms / 1e3, 3),
+            "speedup_vs_single_frame": round(single_ms / ms, 3), "bit_exact_vs_single_frame": exact,
+            "shape": b.shape, "tune": rep.strip().splitlines()}
+
+
 def cpu_baseline(frames, rows, nthreads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -135,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=128)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned block-matching leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-frames leg")
+    ap.add_argument("--only-batched", action="store_true", help="run only the batched leg (prints its JSON)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (three times) and exit; "
                          "used to calibrate the FETCH_SIZE / WRITE_SIZE counters")
@@ -158,6 +228,11 @@ def main():
         return
 
     import bm3d
+
+    if args.only_batched:
+        bm3d.init(0)
+        print(json.dumps(batched_leg()), flush=True)
+        return
 
     frames = make_frames()
     bm3d.init(0)
@@ -244,6 +319,9 @@ def main():
     if not args.no_learned:
         print("learned block-matching leg (4K)", file=sys.stderr, flush=True)
         result["learned_bm"] = learned_leg()
+    if not args.no_batched:
+        print("batched leg (64 uneven Gray frames)", file=sys.stderr, flush=True)
+        result["batched"] = batched_leg()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames, args.cpu_rows, min(FRAMES, os.cpu_count() or 1))
     print(json.dumps(result), flush=True)

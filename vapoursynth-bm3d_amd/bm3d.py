@@ -66,6 +66,10 @@ class Stats(C.Structure):
                 ("members_sq", C.c_int64)]
 
 
+class BatchShape(C.Structure):
+    _fields_ = [("match_waves", C.c_int), ("match_strip", C.c_int), ("grid_per_cu", C.c_int)]
+
+
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
            "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_profile",
            "bm3d_profile_read", "bm3d_debug_matches")
@@ -124,6 +128,18 @@ def load_library(path: str = LIB_PATH):
     L.bm3d_matcher_fallbacks.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.bm3d_match_lists.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int), C.c_void_p, C.c_void_p,
                                    C.c_void_p]
+    L.bm3d_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(Arg), C.c_int, C.c_char_p,
+                                    C.c_size_t]
+    L.bm3d_batch_free.argtypes = [C.c_void_p]
+    L.bm3d_batch_free.restype = None
+    L.bm3d_batch_run.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int,
+                                 C.POINTER(C.c_float)]
+    L.bm3d_batch_tune.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int, C.c_char_p,
+                                  C.c_size_t]
+    L.bm3d_batch_get_shape.argtypes = [C.c_void_p, C.POINTER(BatchShape)]
+    L.bm3d_batch_set_shape.argtypes = [C.c_void_p, C.POINTER(BatchShape)]
+    L.bm3d_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     _lib = L
     return L
 
@@ -519,3 +535,96 @@ def VFinal(input: Clip, ref: Clip, profile: Optional[str] = None, sigma=None, ra
 
 def VAggregate(input: Clip, radius: Optional[int] = None, sample: Optional[int] = None) -> Clip:
     return _node("VAggregate", input, None, {"radius": radius, "sample": sample})
+
+
+# --------------------------------------------------------------------------- batched Basic -> Final
+
+class Batch:
+    """Basic then Final over many Gray float frames of uneven sizes, each
+    with its own sigma, in one persistent launch per stage (no counterpart in
+    the reference, which filters frame by frame).  ``kwargs`` are the
+    Basic/Final arguments shared by every frame (e.g. ``profile="np"``);
+    ``pitches`` (elements) default to the widths.  Frames are float32 torch
+    tensors on the GPU; every output equals Final(Basic(frame)) of the
+    single-frame filters bit for bit."""
+
+    def __init__(self, sizes: Sequence[tuple], sigmas: Sequence[float], pitches: Optional[Sequence[int]] = None,
+                 **kwargs):
+        L = init()
+        self.L = L
+        n = len(sizes)
+        if n == 0 or len(sigmas) != n:
+            raise Error("bm3d.Batch: one sigma per frame is required")
+        self.sizes = [(int(h), int(w)) for h, w in sizes]
+        self.pitches = [int(p) for p in pitches] if pitches is not None else [w for _, w in self.sizes]
+        ws = (C.c_int * n)(*[w for _, w in self.sizes])
+        hs = (C.c_int * n)(*[h for h, _ in self.sizes])
+        ps = (C.c_int * n)(*self.pitches)
+        sg = (C.c_double * n)(*[float(x) for x in sigmas])
+        keep: list = []
+        args, nargs = _make_args(kwargs, keep)
+        handle = C.c_void_p()
+        err = C.create_string_buffer(1024)
+        rc = L.bm3d_batch_create(C.byref(handle), n, ws, hs, ps, sg, args, nargs, err, 1024)
+        if rc != 0:
+            raise Error(err.value.decode() or L.bm3d_last_error().decode())
+        self.h = handle
+        self.n = n
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.L.bm3d_batch_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def _ptrs(self, src, dst):
+        if len(src) != self.n or len(dst) != self.n:
+            raise Error("bm3d.Batch: one source and one destination plane per frame")
+        for k, (s, d) in enumerate(zip(src, dst)):
+            h, w = self.sizes[k]
+            for t in (s, d):
+                if isinstance(t, np.ndarray) or not t.is_cuda or t.dtype.itemsize != 4 or t.dim() != 2:
+                    raise Error("bm3d.Batch: planes must be 2-D float32 GPU tensors")
+                if t.shape[0] != h or t.shape[1] != w or t.stride(1) != 1 or t.stride(0) != self.pitches[k]:
+                    raise Error(f"bm3d.Batch: frame {k}: plane shape/pitch differs from the batch's")
+        sp = (C.c_void_p * self.n)(*[s.data_ptr() for s in src])
+        dp = (C.c_void_p * self.n)(*[d.data_ptr() for d in dst])
+        _sync_producer(src[0].device)
+        return sp, dp
+
+    def run(self, src, dst, graph: bool = False) -> float:
+        """Filters src[k] into dst[k]; returns the chain's GPU time in ms."""
+        sp, dp = self._ptrs(src, dst)
+        ms = C.c_float()
+        if self.L.bm3d_batch_run(self.h, sp, dp, 1 if graph else 0, C.byref(ms)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return ms.value
+
+    def tune(self, src, dst, reps: int = 2) -> str:
+        """Times the supported launch shapes and keeps the fastest; returns the report."""
+        sp, dp = self._ptrs(src, dst)
+        rep = C.create_string_buffer(8192)
+        if self.L.bm3d_batch_tune(self.h, sp, dp, reps, rep, 8192) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return rep.value.decode()
+
+    @property
+    def shape(self) -> Dict[str, int]:
+        s = BatchShape()
+        if self.L.bm3d_batch_get_shape(self.h, C.byref(s)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return dict(match_waves=s.match_waves, match_strip=s.match_strip, grid_per_cu=s.grid_per_cu)
+
+    @shape.setter
+    def shape(self, v: Dict[str, int]) -> None:
+        s = BatchShape(int(v["match_waves"]), int(v["match_strip"]), int(v["grid_per_cu"]))
+        if self.L.bm3d_batch_set_shape(self.h, C.byref(s)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+
+    def info(self) -> Dict[str, int]:
+        b, f, px = C.c_int64(), C.c_int64(), C.c_int64()
+        if self.L.bm3d_batch_info(self.h, C.byref(b), C.byref(f), C.byref(px)) != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        return dict(basic_blocks=b.value, final_blocks=f.value, pixels=px.value)

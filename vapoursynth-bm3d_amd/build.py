@@ -20,7 +20,8 @@ LIB = os.path.join(OUT_DIR, "libbm3d_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BM3D_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bm3d_kernels.hip", "bm3d_vkernels.hip", "lbm_kernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp", "lbm.cpp"]
+SOURCES = ["bm3d_kernels.hip", "bm3d_vkernels.hip", "lbm_kernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp", "lbm.cpp",
+           "bm3d_batch.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -65,15 +66,30 @@ def write_dct_header(path: str, nmax: int = 32) -> None:
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     write_dct_header(os.path.join(OUT_DIR, "bm3d_dct_coefs.h"))
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmd = [HIPCC, *FLAGS, "-I", OUT_DIR, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
-            print(" ".join(shlex.quote(c) for c in cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([HIPCC, *FLAGS, "-I", OUT_DIR, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # translation units compile in parallel (at most 4 hipcc at a time)
+    jobs = max(1, min(4, os.cpu_count() or 1))
+    pending = list(cmds)
+    running = []
+    failed = None
+    while pending or running:
+        while pending and len(running) < jobs and failed is None:
+            cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+        if failed is not None:
+            pending.clear()
+        cmd, proc = running.pop(0)
+        if proc.wait() != 0 and failed is None:
+            failed = subprocess.CalledProcessError(proc.returncode, cmd)
+    if failed is not None:
+        raise failed
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     if verbose:

@@ -65,6 +65,19 @@ struct RgbArgs {
     float m[9];
 };
 
+// One frame of a batched matching stage: its parameters and matching plane.
+struct BatchFrame {
+    StageParams p;
+    const float* plane;
+};
+
+// Tunable launch shapes of the batched stages.
+struct BatchShape {
+    int match_waves;   // reference blocks (waves) per matching workgroup: 1, 2 or 4
+    int match_strip;   // candidates per lane strip in the LDS search tile: 8 or 11
+    int grid_per_cu;   // persistent workgroups per CU for the filter and aggregation
+};
+
 constexpr int kMaxStackFrames = 33;  // 2 * radius + 1 with radius <= 16
 
 struct VMatchArgs {
@@ -119,6 +132,17 @@ bool verify_dct_constants();
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
                         float* key, hipStream_t st, const int* blist = nullptr, const int* nlist = nullptr);
 hipError_t launch_filter(const FilterArgs& a, hipStream_t st);
+// Batched stages over frames numbered by starts[] (device, nframes entries).
+bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape& sh);
+hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int nframes, int total,
+                              const StageParams& p0, int max_cand, const BatchShape& sh, int* count, int* pos,
+                              float* key, hipStream_t st);
+hipError_t launch_filter_batch(const FilterArgs* fa, const int* starts, int nframes, const int* order, int total,
+                               int B, int G, bool wiener, const BatchShape& sh, hipStream_t st);
+hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
+                                  const BatchShape& sh, hipStream_t st);
+int device_cus();
+int aggregate_tiles(int W, int H);
 hipError_t launch_aggregate(const AggArgs& a, hipStream_t st);
 hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
                            float gain, float offset, int clip, float lo, float hi, hipStream_t st);

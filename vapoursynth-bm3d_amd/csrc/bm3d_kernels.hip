@@ -150,27 +150,16 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
 // keys in registers.
 constexpr int kSelCap = 256;
 
-// NW waves (reference blocks) per workgroup.  With blist, wave w of the grid
-// takes reference block blist[w] for w < *nlist (a sparse re-run).
-template <int KMAX, int W, int NW>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8(
-    const float* __restrict__ plane, StageParams p, int nb, int wave_words, const int* __restrict__ blist,
-    const int* __restrict__ nlist, int* __restrict__ out_count, int* __restrict__ out_pos,
-    float* __restrict__ out_key) {
-    extern __shared__ uint32_t smem[];
-    const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
+// Reference block b of one plane (the body of k_match8 and its batched
+// form); wbase: the wave's LDS words.
+template <int KMAX, int W>
+__device__ __forceinline__ void match8_block(const float* __restrict__ plane, const StageParams& p, int b,
+                                             uint32_t* wbase, int* __restrict__ out_count,
+                                             int* __restrict__ out_pos, float* __restrict__ out_key) {
     const int lane = lane_id();
-    int b = blockIdx.x * NW + wave;
-    if (blist) {
-        if (b >= uniform(*nlist)) return;
-        b = uniform(blist[b]);
-    } else if (b >= nb) {
-        return;
-    }
     constexpr int B = 8;
     constexpr int NK = KMAX * W;
     const int G = p.group;
-    uint32_t* wbase = smem + (size_t)wave * wave_words;
     float* win = reinterpret_cast<float*>(wbase);
     const int bj = b / p.nx, bi = b - bj * p.nx;
     const int j = grid_pos(bj, p.block_step, p.height - B);
@@ -413,6 +402,56 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
         const int cy = c / ncols;
         pos_out[1 + e] = ((t + cy) << 16) | (l + c - cy * ncols);
         key_out[1 + e] = __uint_as_float(sel_key[kk + e]);
+    }
+}
+
+// NW waves (reference blocks) per workgroup.  With blist, wave w of the grid
+// takes reference block blist[w] for w < *nlist (a sparse re-run).
+template <int KMAX, int W, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8(
+    const float* __restrict__ plane, StageParams p, int nb, int wave_words, const int* __restrict__ blist,
+    const int* __restrict__ nlist, int* __restrict__ out_count, int* __restrict__ out_pos,
+    float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
+    int b = blockIdx.x * NW + wave;
+    if (blist) {
+        if (b >= uniform(*nlist)) return;
+        b = uniform(blist[b]);
+    } else if (b >= nb) {
+        return;
+    }
+    match8_block<KMAX, W>(plane, p, b, smem + (size_t)wave * wave_words, out_count, out_pos, out_key);
+}
+
+// Frame of global item idx: the last f with starts[f] <= idx (starts[0] = 0,
+// ascending, nframes entries; wave-uniform).
+__device__ __forceinline__ int batch_frame(const int* __restrict__ starts, int nframes, int idx) {
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uniform(starts[mid]) <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Batched form: the reference blocks of every frame of a batch, numbered
+// frame by frame, in one persistent launch (each wave strides over them).
+template <int KMAX, int W, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8_batch(
+    const BatchFrame* __restrict__ frames, const int* __restrict__ starts, int nframes, int total, int wave_words,
+    int G, int* __restrict__ count, int* __restrict__ pos, float* __restrict__ key) {
+    extern __shared__ uint32_t smem[];
+    const int wave = uniform(threadIdx.x >> 6);
+    uint32_t* wbase = smem + (size_t)wave * wave_words;
+    for (int gb = blockIdx.x * NW + wave; gb < total; gb += gridDim.x * NW) {
+        const int f = batch_frame(starts, nframes, gb);
+        const StageParams p = frames[f].p;
+        const float* plane = frames[f].plane;
+        const int s0 = uniform(starts[f]);
+        match8_block<KMAX, W>(plane, p, gb - s0, wbase, count + s0, pos + (size_t)s0 * G, key + (size_t)s0 * G);
+        wave_sync_lds();
     }
 }
 
@@ -925,12 +964,9 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
 
 // One wave per group, taken from `order` (groups bucketed by size so that
 // neighbouring waves run the same specialisation).
+// Group g_id (the body of k_filter8 and its batched form).
 template <int GMAX, bool WIENER>
-__global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
-    extern __shared__ float fsm[];
-    const int slot = a.b0 + blockIdx.x;
-    if (slot >= a.nb) return;
-    const int g_id = a.order ? a.order[slot] : slot;
+__device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, float* fsm) {
     const int lane = lane_id();
     const int G = a.p.group;
     const int n = uniform(a.count[g_id] < G ? a.count[g_id] : G);
@@ -955,6 +991,28 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
 #undef BM3D_G8
     default:
         break;
+    }
+}
+
+template <int GMAX, bool WIENER>
+__global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
+    extern __shared__ float fsm[];
+    const int slot = a.b0 + blockIdx.x;
+    if (slot >= a.nb) return;
+    filter8_group<GMAX, WIENER>(a, a.order ? a.order[slot] : slot, fsm);
+}
+
+// Batched form: the groups of every frame (numbered frame by frame, taken in
+// the batch-wide size order) in one persistent launch.
+template <int GMAX, bool WIENER>
+__global__ __launch_bounds__(64) void k_filter8_batch(const FilterArgs* __restrict__ fa, const int* __restrict__ starts,
+                                                      int nframes, const int* __restrict__ order, int total) {
+    extern __shared__ float fsm[];
+    for (int slot = blockIdx.x; slot < total; slot += gridDim.x) {
+        const int gid = uniform(order[slot]);
+        const int f = batch_frame(starts, nframes, gid);
+        filter8_group<GMAX, WIENER>(fa[f], gid - uniform(starts[f]), fsm);
+        wave_sync_lds();
     }
 }
 
@@ -1084,14 +1142,10 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
     return (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
 }
 
-__global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
-    __shared__ float acc[kAggLds];
-    __shared__ int rec[kAggRecCap * kAggRecWords];
+// Output tile `tile` of one frame (the body of k_aggregate and its batched form).
+__device__ __forceinline__ void aggregate_tile(const AggArgs& a, int tile, float* acc, int* rec) {
     const int lane = lane_id();
     const int tiles_x = (a.p.width + kAggTW - 1) / kAggTW;
-    const int tiles_y = (a.p.height + kAggTH - 1) / kAggTH;
-    const int tile = xcd_tile(tiles_x * tiles_y);
-    if (tile >= tiles_x * tiles_y) return;
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
     const int y0 = ty * kAggTH, x0 = tx * kAggTW;
     int a_lo, a_hi, c_lo, c_hi;
@@ -1112,6 +1166,27 @@ __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
             k += proc[pl] ? 1 : 0;
         }
         write_output(a.out, px, py, v, proc);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
+    __shared__ float acc[kAggLds];
+    __shared__ int rec[kAggRecCap * kAggRecWords];
+    const int ntiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    aggregate_tile(a, tile, acc, rec);
+}
+
+// Batched form: the output tiles of every frame in one persistent launch.
+__global__ __launch_bounds__(64) void k_aggregate_batch(const AggArgs* __restrict__ aa, const int* __restrict__ starts,
+                                                        int nframes, int total) {
+    __shared__ float acc[kAggLds];
+    __shared__ int rec[kAggRecCap * kAggRecWords];
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int f = batch_frame(starts, nframes, t);
+        aggregate_tile(aa[f], t - uniform(starts[f]), acc, rec);
+        wave_sync_lds();
     }
 }
 
@@ -1192,6 +1267,16 @@ struct Match8Shape {
     int w, kmax;
     size_t words;
 };
+
+// Shape of the B = 8 fast path for strip width w.
+static Match8Shape match8_shape_w(int max_cand, int G, int w) {
+    const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
+    const int nstr = (side + w - 1) / w;
+    const int kmax = (side * nstr + 63) / 64;
+    const size_t ww = (size_t)((nstr * w + 7) | 1);
+    const size_t win = ((size_t)(side + 7) * ww + 3) & ~(size_t)3;
+    return {w, kmax, win + 64 + 2 * (size_t)kSelCap + 4 * (size_t)G};
+}
 
 Match8Shape match8_shape(int max_cand, int G) {
     const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
@@ -1319,6 +1404,88 @@ hipError_t launch_aggregate(const AggArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_aggregate, dim3(8 * ((tiles + 7) / 8)), dim3(64), 0, st, a);
     return hipGetLastError();
 }
+
+int device_cus() {
+    static int n = [] {
+        int d = 0, v = 0;
+        if (hipGetDevice(&d) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || v <= 0) return 256;
+        return v;
+    }();
+    return n;
+}
+
+bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape& sh) {
+    if (p.block != 8 || p.step != 1) return false;
+    if (sh.match_strip != 8 && sh.match_strip != 11) return false;
+    if (sh.match_waves != 1 && sh.match_waves != 2 && sh.match_waves != 4) return false;
+    const Match8Shape ms = match8_shape_w(max_cand, p.group, sh.match_strip);
+    return ms.kmax >= 1 && ms.kmax <= 4 && ms.words * 4 * sh.match_waves <= 65536;
+}
+
+hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int nframes, int total,
+                              const StageParams& p0, int max_cand, const BatchShape& sh, int* count, int* pos,
+                              float* key, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    if (!batch_match_supported(p0, max_cand, sh)) return hipErrorInvalidValue;
+    const Match8Shape ms = match8_shape_w(max_cand, p0.group, sh.match_strip);
+    const int km = ms.kmax <= 1 ? 1 : ms.kmax <= 2 ? 2 : ms.kmax <= 3 ? 3 : 4;
+    const int nw = sh.match_waves;
+    const size_t lds = ms.words * 4 * nw;
+    int groups = device_cus() * (16 / nw);  // persistent: 16 waves per CU
+    const int need = (total + nw - 1) / nw;
+    groups = groups < need ? groups : need;
+#define BM3D_M8B(K, WD, NW)                                                                                  \
+    if (km == K && ms.w == WD && nw == NW) {                                                                 \
+        hipLaunchKernelGGL((k_match8_batch<K, WD, NW>), dim3(groups), dim3(64 * NW), lds, st, frames, starts, \
+                           nframes, total, (int)ms.words, p0.group, count, pos, key);                        \
+        return hipGetLastError();                                                                            \
+    }
+#define BM3D_M8BW(NW) \
+    BM3D_M8B(1, 8, NW) BM3D_M8B(2, 8, NW) BM3D_M8B(3, 8, NW) BM3D_M8B(4, 8, NW) \
+    BM3D_M8B(1, 11, NW) BM3D_M8B(2, 11, NW) BM3D_M8B(3, 11, NW) BM3D_M8B(4, 11, NW)
+    BM3D_M8BW(1)
+    BM3D_M8BW(2)
+    BM3D_M8BW(4)
+#undef BM3D_M8BW
+#undef BM3D_M8B
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_filter_batch(const FilterArgs* fa, const int* starts, int nframes, const int* order, int total,
+                               int B, int G, bool wiener, const BatchShape& sh, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    const int gm = filter8_gmax(G);
+    if (B != 8 || gm == 0) return hipErrorInvalidValue;
+    const size_t lds = filter8_lds_bytes(G, wiener);
+    int groups = device_cus() * sh.grid_per_cu;
+    groups = groups < total ? groups : total;
+#define BM3D_F8B(GM, WI)                                                                                   \
+    if (gm == GM && wiener == WI) {                                                                         \
+        hipLaunchKernelGGL((k_filter8_batch<GM, WI>), dim3(groups), dim3(64), lds, st, fa, starts, nframes, \
+                           order, total);                                                                   \
+        return hipGetLastError();                                                                           \
+    }
+    BM3D_F8B(8, false)
+    BM3D_F8B(16, false)
+    BM3D_F8B(32, false)
+    BM3D_F8B(8, true)
+    BM3D_F8B(16, true)
+    BM3D_F8B(32, true)
+#undef BM3D_F8B
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
+                                  const BatchShape& sh, hipStream_t st) {
+    if (total <= 0) return hipSuccess;
+    int groups = device_cus() * sh.grid_per_cu;
+    groups = groups < total ? groups : total;
+    hipLaunchKernelGGL(k_aggregate_batch, dim3(groups), dim3(64), 0, st, aa, starts, nframes, total);
+    return hipGetLastError();
+}
+
+int aggregate_tiles(int W, int H) { return ((W + kAggTW - 1) / kAggTW) * ((H + kAggTH - 1) / kAggTH); }
 
 hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
                            float gain, float offset, int clip, float lo, float hi, hipStream_t st) {
