@@ -14,7 +14,7 @@ one frame per thread like VapourSynth's fmParallel) on a bounded sample.
 learned_bm = the secondary metric: learned-proposal vs exhaustive block
 matching on a 4K Gray frame (speedup, recall, PSNR loss).  batched = Basic ->
 Final over 64 Gray float frames of uneven sizes with per-frame sigma, one
-persistent launch per stage, HIP-graph replay, against the single-frame
+launch per stage over all frames, HIP-graph replay, against the single-frame
 filters.
 """
 from __future__ import annotations
@@ -129,7 +129,7 @@ def batch_workload(n=BATCH_FRAMES, seed=64):
 
 def batched_leg(reps=3, tune=True):
     """Batched Basic(np) -> Final(np) over the 64-frame uneven workload: one
-    persistent launch per stage for all frames, the chain captured in a HIP
+    launch per stage over all frames, the chain captured in a HIP
     graph after autotuning the launch shape.  Compared with the same frames
     filtered one at a time through the single-frame filters (steady state,
     each frame's filters created and warmed up before its timed call), and

@@ -8,7 +8,7 @@
  * its own default matrix and thresholds, exactly as a separate Basic/Final
  * call would derive them, source/BM3D_Base.cpp:106-236); the remaining
  * arguments (profile, block_size, ...) are shared.  Every stage -- block
- * matching, collaborative filtering, aggregation -- runs as one persistent
+ * matching, collaborative filtering, aggregation -- runs as one
  * launch over the work of all frames; the Basic -> Final chain can be
  * captured once into a HIP graph and replayed.  Each output equals
  * Final(Basic(frame)) of the single-frame filters bit for bit.
@@ -34,7 +34,7 @@ typedef struct bm3d_batch bm3d_batch;
 typedef struct bm3d_batch_shape {
     int match_waves;  /* reference blocks (waves) per matching workgroup: 1, 2 or 4 */
     int match_strip;  /* candidates per lane strip of the LDS search tile: 8 or 11 */
-    int grid_per_cu;  /* persistent workgroups per CU of the filter and aggregation */
+    int group_order;  /* 0: groups bucketed by size per frame; 1: per chunk of 4096 groups */
 } bm3d_batch_shape;
 
 /* widths, heights, pitches (elements), sigmas: nframes entries each.  args:

@@ -1,5 +1,5 @@
 """GPU tests of the batched Basic -> Final path (many Gray float frames of
-uneven sizes, one sigma per frame, one persistent launch per stage).
+uneven sizes, one sigma per frame, one launch per stage over all frames).
 
 Bar: bit-exact.  Every batched output must equal Final(Basic(frame)) of the
 single-frame filters, and on small frames the oracle's chain; HIP-graph
@@ -115,9 +115,11 @@ def test_autotune_keeps_results(gpu):
     rep = b.tune(src, _outputs(FRAMES), reps=1)
     assert "best" in rep
     lines = [l for l in rep.splitlines() if l.startswith("match_waves")]
-    assert len(lines) >= 4, rep
-    for shape in ({"match_waves": 1, "match_strip": 8, "grid_per_cu": 4},
-                  {"match_waves": 4, "match_strip": 11, "grid_per_cu": 16}):
+    assert len(lines) >= 6, rep
+    for shape in ({"match_waves": 1, "match_strip": 8, "group_order": 0},
+                  {"match_waves": 4, "match_strip": 11, "group_order": 1},
+                  {"match_waves": 2, "match_strip": 11, "group_order": 0},
+                  {"match_waves": 2, "match_strip": 8, "group_order": 1}):
         b.shape = shape
         assert b.shape == shape
         out = _outputs(FRAMES)
@@ -126,8 +128,23 @@ def test_autotune_keeps_results(gpu):
             _bits_equal(_host(out[k]), want[k], f"shape {shape} frame {k}")
 
 
+@pytest.mark.parametrize("order", [0, 1])
+def test_large_frames_both_group_orders(gpu, order):
+    # frames of more than one 4096-group chunk: the bucketing takes both its
+    # single-frame-chunk and its frame-spanning-chunk paths
+    frames = [(256, 256, 9.0), (40, 56, 21.0), (200, 330, 14.0), (24, 40, 6.0)]
+    hosts, src = _inputs(frames, 7500)
+    b = gpu.Batch([(h, w) for h, w, _ in frames], [s for *_, s in frames], profile="np")
+    for nw in (1, 4):
+        b.shape = {"match_waves": nw, "match_strip": 11, "group_order": order}
+        dst = _outputs(frames)
+        b.run(src, dst, graph=True)
+        for k, (h, w, s) in enumerate(frames):
+            _bits_equal(_host(dst[k]), _single(gpu, hosts[k], s, profile="np"), f"order {order} waves {nw} frame {k}")
+
+
 def test_per_frame_fallback_shapes(gpu):
-    # bm_step 2 and block_size 4 are outside the 8x8 persistent kernels:
+    # bm_step 2 and block_size 4 are outside the 8x8 batched kernels:
     # the batch runs one single-frame launch per frame and stage
     frames = FRAMES[:4]
     for kw in (dict(profile="fast", bm_step=2), dict(profile="lc", block_size=4, block_step=3)):
@@ -153,4 +170,6 @@ def test_batch_errors(gpu):
         gpu.Batch([(32, 32), (6, 40)], [10.0, 10.0])
     b = gpu.Batch([(32, 32)], [10.0])
     with pytest.raises(gpu.Error, match="shape"):
-        b.shape = {"match_waves": 3, "match_strip": 8, "grid_per_cu": 8}
+        b.shape = {"match_waves": 3, "match_strip": 8, "group_order": 0}
+    with pytest.raises(gpu.Error, match="shape"):
+        b.shape = {"match_waves": 2, "match_strip": 8, "group_order": 2}

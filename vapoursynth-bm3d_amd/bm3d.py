@@ -67,7 +67,7 @@ class Stats(C.Structure):
 
 
 class BatchShape(C.Structure):
-    _fields_ = [("match_waves", C.c_int), ("match_strip", C.c_int), ("grid_per_cu", C.c_int)]
+    _fields_ = [("match_waves", C.c_int), ("match_strip", C.c_int), ("group_order", C.c_int)]
 
 
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
@@ -541,7 +541,7 @@ def VAggregate(input: Clip, radius: Optional[int] = None, sample: Optional[int] 
 
 class Batch:
     """Basic then Final over many Gray float frames of uneven sizes, each
-    with its own sigma, in one persistent launch per stage (no counterpart in
+    with its own sigma, in one launch per stage (no counterpart in
     the reference, which filters frame by frame).  ``kwargs`` are the
     Basic/Final arguments shared by every frame (e.g. ``profile="np"``);
     ``pitches`` (elements) default to the widths.  Frames are float32 torch
@@ -615,11 +615,11 @@ class Batch:
         s = BatchShape()
         if self.L.bm3d_batch_get_shape(self.h, C.byref(s)) != 0:
             raise Error(self.L.bm3d_last_error().decode())
-        return dict(match_waves=s.match_waves, match_strip=s.match_strip, grid_per_cu=s.grid_per_cu)
+        return dict(match_waves=s.match_waves, match_strip=s.match_strip, group_order=s.group_order)
 
     @shape.setter
     def shape(self, v: Dict[str, int]) -> None:
-        s = BatchShape(int(v["match_waves"]), int(v["match_strip"]), int(v["grid_per_cu"]))
+        s = BatchShape(int(v["match_waves"]), int(v["match_strip"]), int(v["group_order"]))
         if self.L.bm3d_batch_set_shape(self.h, C.byref(s)) != 0:
             raise Error(self.L.bm3d_last_error().decode())
 

@@ -5,7 +5,7 @@
 // tables are exactly those of a single-frame call, source/BM3D_Base.cpp:
 // 106-236).  The work of all frames is numbered frame by frame -- reference
 // blocks for matching and filtering, 32x16 output tiles for aggregation --
-// and each stage is one persistent launch whose waves look up the frame of
+// and each stage is one launch over all frames whose waves look up the frame of
 // the item they take (starts[] arrays in device memory).  The per-frame
 // argument blocks (plane pointers, stage parameters, output planes) live in
 // one device blob that is rewritten before a run only when the caller's
@@ -44,12 +44,14 @@ struct Stage {
     int total = 0;                      // reference blocks of the batch
     int G = 0, B = 0, max_cand = 0;
     bool wiener = false;
-    bool batched = false;               // 8x8 persistent kernels apply
+    bool batched = false;               // the 8x8 batched kernels apply
     // offsets (bytes) inside the device blob
-    size_t off_start = 0, off_frames = 0, off_fa = 0, off_ag = 0, off_ptrs = 0;
+    size_t off_start = 0, off_frames = 0, off_ff = 0, off_ag = 0, off_ptrs = 0;
     std::vector<BatchFrame> frames;     // host images of the blob sections
-    std::vector<FilterArgs> fa;
+    std::vector<BatchFilterFrame> ff;
     std::vector<AggArgs> ag;
+    std::vector<FilterArgs> fa;         // per-frame filter arguments (per-frame launches)
+    FilterArgs a0{};                    // shared filter arguments (batched launch)
     std::vector<const float*> ptrs;     // 6 per frame: src p0..p2, ref p0..p2
 };
 
@@ -123,6 +125,8 @@ void fill_stage(bm3d_batch* b, int which, const float* const* src, float* const*
         pt[4] = pt[5] = nullptr;
         const float* const* dpt =
             reinterpret_cast<const float* const*>(b->blob.as<char>() + s.off_ptrs) + 6 * f;
+        const bm3d_filter* flt = s.filt[f].get();
+        s.ff[f] = BatchFilterFrame{b->pitch[f], dpt, flt->tables_dev.as<PlaneTables>()};
         FilterArgs& fa = s.fa[f];
         fa = FilterArgs{};
         fa.p = s.sp[f];
@@ -138,7 +142,6 @@ void fill_stage(bm3d_batch* b, int which, const float* const* src, float* const*
             fa.src[p] = dpt + p;
             fa.ref[p] = dpt + 3 + p;
         }
-        const bm3d_filter* flt = s.filt[f].get();
         fa.tables = flt->tables_dev.as<PlaneTables>();
         fa.rcp_table = rcp_dev();
         fa.rcp_shift = (s.wiener && rcp_bits() == 0) ? 23 : 23 - rcp_bits();
@@ -160,6 +163,15 @@ void fill_stage(bm3d_batch* b, int which, const float* const* src, float* const*
         ag.out.dst[0] = which == 0 ? static_cast<void*>(est + b->est_off[f]) : static_cast<void*>(dst[f]);
         ag.out.pitch[0] = (size_t)b->pitch[f];
     }
+    // batched launch: the stage-wide arrays indexed by the batch-wide group number
+    FilterArgs& a0 = s.a0;
+    a0 = s.fa[0];
+    a0.nb = s.total;
+    a0.count = b->count.as<int>();
+    a0.pos = b->pos.as<int>();
+    a0.order = nullptr;
+    a0.contrib[0] = b->contrib.as<float>();
+    a0.denw[0] = b->denw.as<float>();
 }
 
 // Writes the blob image on the host and uploads it (ordered on the stream).
@@ -171,7 +183,7 @@ int upload_blob(bm3d_batch* b, const float* const* src, float* const* dst) {
         Stage& s = b->st[which];
         std::memcpy(h + s.off_start, s.start.data(), sizeof(int) * b->n);
         std::memcpy(h + s.off_frames, s.frames.data(), sizeof(BatchFrame) * b->n);
-        std::memcpy(h + s.off_fa, s.fa.data(), sizeof(FilterArgs) * b->n);
+        std::memcpy(h + s.off_ff, s.ff.data(), sizeof(BatchFilterFrame) * b->n);
         std::memcpy(h + s.off_ag, s.ag.data(), sizeof(AggArgs) * b->n);
         std::memcpy(h + s.off_ptrs, s.ptrs.data(), sizeof(float*) * 6 * b->n);
     }
@@ -200,9 +212,10 @@ int enqueue_stage(bm3d_batch* b, int which) {
         const int* starts = blob_at<int>(b, s.off_start);
         HIP_TRY(launch_match_batch(blob_at<BatchFrame>(b, s.off_frames), starts, b->n, s.total, s.sp[0], s.max_cand,
                                    b->shape, cnt, pos, key, st));
-        HIP_TRY(launch_bucket(cnt, s.total, s.G, b->hist.as<int>(), b->order.as<int>(), st));
-        HIP_TRY(launch_filter_batch(blob_at<FilterArgs>(b, s.off_fa), starts, b->n, b->order.as<int>(), s.total, s.B,
-                                    s.G, s.wiener, b->shape, st));
+        HIP_TRY(launch_bucket_batch(cnt, starts, b->n, s.total, s.G, b->shape.group_order, b->hist.as<int>(),
+                                    b->order.as<int>(), st));
+        HIP_TRY(launch_filter_batch(s.a0, blob_at<BatchFilterFrame>(b, s.off_ff), starts, b->n, b->order.as<int>(),
+                                    s.total, b->shape, st));
         HIP_TRY(launch_aggregate_batch(blob_at<AggArgs>(b, s.off_ag), blob_at<int>(b, b->off_tstart), b->n, b->ttotal,
                                        b->shape, st));
         return BM3D_OK;
@@ -268,13 +281,13 @@ int run_chain(bm3d_batch* b, const float* const* src, float* const* dst, bool gr
 
 std::string shape_name(const BatchShape& s) {
     char buf[96];
-    std::snprintf(buf, sizeof(buf), "match_waves=%d match_strip=%d grid_per_cu=%d", s.match_waves, s.match_strip,
-                  s.grid_per_cu);
+    std::snprintf(buf, sizeof(buf), "match_waves=%d match_strip=%d group_order=%d", s.match_waves, s.match_strip,
+                  s.group_order);
     return buf;
 }
 
 bool shape_ok(const bm3d_batch* b, const BatchShape& sh) {
-    if (sh.grid_per_cu < 1 || sh.grid_per_cu > 64) return false;
+    if (sh.group_order != kOrderFrame && sh.group_order != kOrderChunk) return false;
     for (const Stage& s : b->st)
         if (s.batched && !batch_match_supported(s.sp[0], s.max_cand, sh)) return false;
     return true;
@@ -352,15 +365,16 @@ int bm3d_batch_create(bm3d_batch** out, int nframes, const int* widths, const in
             return set_error(BM3D_E_UNSUPPORTED,
                              "bm3d_batch_create: block_size/group_size too large for the on-chip group buffer");
         s.frames.resize(nframes);
+        s.ff.resize(nframes);
         s.fa.resize(nframes);
         s.ag.resize(nframes);
         s.ptrs.assign(6 * (size_t)nframes, nullptr);
     }
-    // default launch shape: the single-frame matcher's strip width, 2 waves
-    // per matching workgroup, 8 persistent workgroups per CU
-    b->shape = BatchShape{2, 11, 8};
+    // default launch shape: 2 waves per matching workgroup, the wider strip
+    // when it fits, size buckets per chunk of groups
+    b->shape = BatchShape{2, 11, kOrderChunk};
     for (int w : {11, 8}) {
-        BatchShape t{2, w, 8};
+        BatchShape t{2, w, kOrderChunk};
         bool ok = true;
         for (const Stage& s : b->st)
             ok = ok && s.B == 8 && s.sp[0].step == 1 && batch_match_supported(s.sp[0], s.max_cand, t);
@@ -395,7 +409,7 @@ int bm3d_batch_create(bm3d_batch** out, int nframes, const int* widths, const in
     for (Stage& s : b->st) {
         s.off_start = take(sizeof(int) * nframes);
         s.off_frames = take(sizeof(BatchFrame) * nframes);
-        s.off_fa = take(sizeof(FilterArgs) * nframes);
+        s.off_ff = take(sizeof(BatchFilterFrame) * nframes);
         s.off_ag = take(sizeof(AggArgs) * nframes);
         s.off_ptrs = take(sizeof(float*) * 6 * nframes);
     }
@@ -416,7 +430,7 @@ int bm3d_batch_create(bm3d_batch** out, int nframes, const int* widths, const in
     if ((rc = b->pos.ensure(sizeof(int) * maxbg))) return rc;
     if ((rc = b->key.ensure(sizeof(float) * maxbg))) return rc;
     if ((rc = b->order.ensure(sizeof(int) * maxb))) return rc;
-    if ((rc = b->hist.ensure(sizeof(int) * (size_t)(gmax + 1)))) return rc;
+    if ((rc = b->hist.ensure(sizeof(int) * (size_t)nframes * (gmax + 1)))) return rc;
     if ((rc = b->contrib.ensure(sizeof(float) * maxc))) return rc;
     if ((rc = b->denw.ensure(sizeof(float) * maxb))) return rc;
     HIP_TRY(hipStreamCreate(&b->stream));  // ordered with the legacy default stream
@@ -439,13 +453,13 @@ int bm3d_batch_get_shape(const bm3d_batch* b, bm3d_batch_shape* shape) {
     if (!b || !shape) return set_error(BM3D_E_STATE, "bm3d_batch_get_shape: null argument");
     shape->match_waves = b->shape.match_waves;
     shape->match_strip = b->shape.match_strip;
-    shape->grid_per_cu = b->shape.grid_per_cu;
+    shape->group_order = b->shape.group_order;
     return BM3D_OK;
 }
 
 int bm3d_batch_set_shape(bm3d_batch* b, const bm3d_batch_shape* shape) {
     if (!b || !shape) return set_error(BM3D_E_STATE, "bm3d_batch_set_shape: null argument");
-    BatchShape t{shape->match_waves, shape->match_strip, shape->grid_per_cu};
+    BatchShape t{shape->match_waves, shape->match_strip, shape->group_order};
     if (!shape_ok(b, t)) return set_error(BM3D_E_ARG, "bm3d_batch_set_shape: unsupported shape " + shape_name(t));
     b->shape = t;
     b->drop_graph();
@@ -477,10 +491,10 @@ int bm3d_batch_tune(bm3d_batch* b, const float* const* src, float* const* dst, i
     float best_ms = -1.0f;
     int rc;
     if (any_batched) {
-        // matching shape first (at the default grid), then the persistent grid
+        // matching shape first (at the default group order), then the order
         for (int nw : {1, 2, 4})
             for (int w : {8, 11}) {
-                BatchShape t{nw, w, best.grid_per_cu};
+                BatchShape t{nw, w, best.group_order};
                 if (!shape_ok(b, t)) continue;
                 float ms;
                 if ((rc = time_shape(t, ms))) return rc;
@@ -489,16 +503,13 @@ int bm3d_batch_tune(bm3d_batch* b, const float* const* src, float* const* dst, i
                     best = t;
                 }
             }
-        const BatchShape mbest = best;
-        for (int g : {4, 6, 12, 16, 24}) {
-            BatchShape t{mbest.match_waves, mbest.match_strip, g};
-            if (!shape_ok(b, t)) continue;
-            float ms;
-            if ((rc = time_shape(t, ms))) return rc;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = t;
-            }
+        BatchShape t = best;
+        t.group_order = best.group_order == kOrderFrame ? kOrderChunk : kOrderFrame;
+        float ms;
+        if ((rc = time_shape(t, ms))) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
         }
     }
     b->shape = best;

@@ -71,11 +71,21 @@ struct BatchFrame {
     const float* plane;
 };
 
+// Per-frame part of a batched filter stage.
+struct BatchFilterFrame {
+    int pitch;                   // elements
+    const float* const* planes;  // device table: src p0..p2, ref p0..p2
+    const PlaneTables* tables;   // device array, one per plane
+};
+
+// Group orders of a batched filter stage.
+enum { kOrderFrame = 0, kOrderChunk = 1 };
+
 // Tunable launch shapes of the batched stages.
 struct BatchShape {
     int match_waves;   // reference blocks (waves) per matching workgroup: 1, 2 or 4
     int match_strip;   // candidates per lane strip in the LDS search tile: 8 or 11
-    int grid_per_cu;   // persistent workgroups per CU for the filter and aggregation
+    int group_order;   // kOrderFrame: size buckets per frame; kOrderChunk: per chunk of groups
 };
 
 constexpr int kMaxStackFrames = 33;  // 2 * radius + 1 with radius <= 16
@@ -137,8 +147,10 @@ bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape&
 hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int nframes, int total,
                               const StageParams& p0, int max_cand, const BatchShape& sh, int* count, int* pos,
                               float* key, hipStream_t st);
-hipError_t launch_filter_batch(const FilterArgs* fa, const int* starts, int nframes, const int* order, int total,
-                               int B, int G, bool wiener, const BatchShape& sh, hipStream_t st);
+hipError_t launch_filter_batch(const FilterArgs& a0, const BatchFilterFrame* ff, const int* starts, int nframes,
+                               const int* order, int total, const BatchShape& sh, hipStream_t st);
+hipError_t launch_bucket_batch(const int* count, const int* starts, int nframes, int nb, int G, int mode, int* hist,
+                               int* order, hipStream_t st);
 hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
                                   const BatchShape& sh, hipStream_t st);
 int device_cus();

@@ -436,23 +436,45 @@ __device__ __forceinline__ int batch_frame(const int* __restrict__ starts, int n
     return lo;
 }
 
+// The same search for a per-lane index.
+__device__ __forceinline__ int lane_frame(const int* __restrict__ starts, int nframes, int idx) {
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Item of consumer `sub` (a wave) of this workgroup in a launch of
+// 8 * ceil(ceil(total / per) / 8) workgroups with `per` consumers each, or -1.
+// Workgroup g runs on XCD g % 8, so the items are split into eight contiguous
+// ranges, one per XCD: neighbouring reference blocks and tiles, which read
+// overlapping pixels and contributions, share an L2.
+__device__ __forceinline__ int xcd_item(int total, int per, int sub) {
+    const int span = (total + 7) >> 3;
+    const int x = (int)(blockIdx.x & 7);
+    const int i = x * span + (int)(blockIdx.x >> 3) * per + sub;
+    return i < min(total, (x + 1) * span) ? i : -1;
+}
+
 // Batched form: the reference blocks of every frame of a batch, numbered
-// frame by frame, in one persistent launch (each wave strides over them).
+// frame by frame, in one launch (one wave per block).
 template <int KMAX, int W, int NW>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8_batch(
     const BatchFrame* __restrict__ frames, const int* __restrict__ starts, int nframes, int total, int wave_words,
     int G, int* __restrict__ count, int* __restrict__ pos, float* __restrict__ key) {
     extern __shared__ uint32_t smem[];
     const int wave = uniform(threadIdx.x >> 6);
-    uint32_t* wbase = smem + (size_t)wave * wave_words;
-    for (int gb = blockIdx.x * NW + wave; gb < total; gb += gridDim.x * NW) {
-        const int f = batch_frame(starts, nframes, gb);
-        const StageParams p = frames[f].p;
-        const float* plane = frames[f].plane;
-        const int s0 = uniform(starts[f]);
-        match8_block<KMAX, W>(plane, p, gb - s0, wbase, count + s0, pos + (size_t)s0 * G, key + (size_t)s0 * G);
-        wave_sync_lds();
-    }
+    const int gb = xcd_item(total, NW, wave);
+    if (gb < 0) return;
+    const int f = batch_frame(starts, nframes, gb);
+    const StageParams p = frames[f].p;
+    const float* plane = frames[f].plane;
+    const int s0 = uniform(starts[f]);
+    match8_block<KMAX, W>(plane, p, gb - s0, smem + (size_t)wave * wave_words, count + s0, pos + (size_t)s0 * G,
+                          key + (size_t)s0 * G);
 }
 
 // ---------------------------------------------------------------------------
@@ -867,13 +889,15 @@ __device__ __forceinline__ void gather8(float* g, const float* const* planes, co
     wave_sync_lds();
 }
 
-template <int N, bool WIENER>
+// ONE: only plane 0 is processed (Gray, or YUV with the chroma planes
+// passed through), which keeps the other planes' pointers out of registers.
+template <int N, bool WIENER, bool ONE>
 __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int* zyx, float* gs) {
     const int lane = lane_id();
     const int G = a.p.group;
     const int y = lane >> 3, x = lane & 7;
     const int off = y * 9 + x;
-    for (int pl = 0; pl < 3; ++pl) {
+    for (int pl = 0; pl < (ONE ? 1 : 3); ++pl) {
         if (!pick3(a.process, pl)) continue;
         const float* const* srcp = pick3(a.src, pl);
         const float* const* refp = pick3(a.ref, pl);
@@ -965,7 +989,7 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
 // One wave per group, taken from `order` (groups bucketed by size so that
 // neighbouring waves run the same specialisation).
 // Group g_id (the body of k_filter8 and its batched form).
-template <int GMAX, bool WIENER>
+template <int GMAX, bool WIENER, bool ONE>
 __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, float* fsm) {
     const int lane = lane_id();
     const int G = a.p.group;
@@ -982,7 +1006,7 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
     switch (n) {
 #define BM3D_G8(NN)                                                         \
     case NN:                                                                \
-        if constexpr (NN <= GMAX) group8<NN, WIENER>(a, g_id, zyx, gs);     \
+        if constexpr (NN <= GMAX) group8<NN, WIENER, ONE>(a, g_id, zyx, gs); \
         break;
         BM3D_G8(1) BM3D_G8(2) BM3D_G8(3) BM3D_G8(4) BM3D_G8(5) BM3D_G8(6) BM3D_G8(7) BM3D_G8(8)
         BM3D_G8(9) BM3D_G8(10) BM3D_G8(11) BM3D_G8(12) BM3D_G8(13) BM3D_G8(14) BM3D_G8(15) BM3D_G8(16)
@@ -994,26 +1018,35 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
     }
 }
 
-template <int GMAX, bool WIENER>
+template <int GMAX, bool WIENER, bool ONE>
 __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
     const int slot = a.b0 + blockIdx.x;
     if (slot >= a.nb) return;
-    filter8_group<GMAX, WIENER>(a, a.order ? a.order[slot] : slot, fsm);
+    filter8_group<GMAX, WIENER, ONE>(a, a.order ? a.order[slot] : slot, fsm);
 }
 
 // Batched form: the groups of every frame (numbered frame by frame, taken in
-// the batch-wide size order) in one persistent launch.
+// `order`) in one launch, one workgroup per group (XCD-contiguous slots).  a0
+// holds what all frames share, with count, pos, contrib and denw indexed by
+// the batch-wide group number; ff[f] the frame's pitch, plane pointer table
+// (src p0..p2, ref p0..p2) and tables.  Plane 0 only (Gray frames).
 template <int GMAX, bool WIENER>
-__global__ __launch_bounds__(64) void k_filter8_batch(const FilterArgs* __restrict__ fa, const int* __restrict__ starts,
-                                                      int nframes, const int* __restrict__ order, int total) {
+__global__ __launch_bounds__(64) void k_filter8_batch(FilterArgs a0, const BatchFilterFrame* __restrict__ ff,
+                                                      const int* __restrict__ starts, int nframes,
+                                                      const int* __restrict__ order, int total) {
     extern __shared__ float fsm[];
-    for (int slot = blockIdx.x; slot < total; slot += gridDim.x) {
-        const int gid = uniform(order[slot]);
-        const int f = batch_frame(starts, nframes, gid);
-        filter8_group<GMAX, WIENER>(fa[f], gid - uniform(starts[f]), fsm);
-        wave_sync_lds();
-    }
+    const int slot = xcd_item(total, 1, 0);
+    if (slot < 0) return;
+    const int gid = uniform(order[slot]);
+    const int f = batch_frame(starts, nframes, gid);
+    FilterArgs a = a0;
+    a.p.pitch = uniform(ff[f].pitch);
+    const float* const* pt = ff[f].planes;
+    a.src[0] = pt;
+    a.ref[0] = pt + 3;
+    a.tables = ff[f].tables;
+    filter8_group<GMAX, WIENER, true>(a, gid, fsm);
 }
 
 // Bucketing of groups by size: histogram, exclusive scan (largest n first),
@@ -1082,12 +1115,147 @@ __global__ __launch_bounds__(kBucketThreads) void k_size_scatter(const int* __re
         if (size[k] >= 1) order[base[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
 }
 
+// Batched bucketing, frame-major: groups of frame 0 first, inside a frame the
+// largest sizes first.  Bin of (frame f, size n): f * (G + 1) + G - n, so an
+// exclusive scan in bin order gives every bin's first slot.  A chunk inside
+// one frame histograms in LDS; a chunk spanning frames goes to the global
+// bins lane by lane.
+__global__ __launch_bounds__(kBucketThreads) void k_fsize_hist(const int* __restrict__ count,
+                                                               const int* __restrict__ starts, int nframes, int nb,
+                                                               int G, int* hist) {
+    __shared__ int h[kMaxGroup + 1];
+    const int lo = blockIdx.x * kBucketChunk;
+    const int hi = min(nb, lo + kBucketChunk);
+    const int f0 = batch_frame(starts, nframes, lo), f1 = batch_frame(starts, nframes, hi - 1);
+    if (f0 != f1) {
+        for (int b = lo + threadIdx.x; b < hi; b += kBucketThreads) {
+            const int n = min(max(count[b], 1), G);
+            atomicAdd(&hist[lane_frame(starts, nframes, b) * (G + 1) + G - n], 1);
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
+    __syncthreads();
+    for (int b = lo + threadIdx.x; b < hi; b += kBucketThreads) atomicAdd(&h[min(max(count[b], 1), G)], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads)
+        if (h[i]) atomicAdd(&hist[f0 * (G + 1) + G - i], h[i]);
+}
+
+// Exclusive prefix over nbins counters, in place (one workgroup).
+__global__ __launch_bounds__(256) void k_bins_scan(int* hist, int nbins) {
+    __shared__ int part[256];
+    const int per = (nbins + 255) / 256;
+    const int a = threadIdx.x * per, e = min(nbins, a + per);
+    int sum = 0;
+    for (int i = a; i < e; ++i) sum += hist[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int i = 0; i < 256; ++i) {
+            const int t = part[i];
+            part[i] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+    int acc = part[threadIdx.x];
+    for (int i = a; i < e; ++i) {
+        const int t = hist[i];
+        hist[i] = acc;
+        acc += t;
+    }
+}
+
+__global__ __launch_bounds__(kBucketThreads) void k_fsize_scatter(const int* __restrict__ count,
+                                                                  const int* __restrict__ starts, int nframes,
+                                                                  int nb, int G, int* cursor, int* order) {
+    __shared__ int h[kMaxGroup + 1];
+    __shared__ int base[kMaxGroup + 1];
+    const int lo = blockIdx.x * kBucketChunk;
+    const int hi = min(nb, lo + kBucketChunk);
+    const int f0 = batch_frame(starts, nframes, lo), f1 = batch_frame(starts, nframes, hi - 1);
+    if (f0 != f1) {
+        for (int b = lo + threadIdx.x; b < hi; b += kBucketThreads) {
+            const int n = min(max(count[b], 1), G);
+            order[atomicAdd(&cursor[lane_frame(starts, nframes, b) * (G + 1) + G - n], 1)] = b;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
+    __syncthreads();
+    int rank[kBucketPerThread], size[kBucketPerThread];
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k) {
+        const int b = lo + threadIdx.x + k * kBucketThreads;
+        size[k] = b < hi ? min(max(count[b], 1), G) : -1;
+        rank[k] = size[k] >= 0 ? atomicAdd(&h[size[k]], 1) : 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads)
+        base[i] = h[i] ? atomicAdd(&cursor[f0 * (G + 1) + G - i], h[i]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k)
+        if (size[k] >= 1) order[base[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
+}
+
+// Batched bucketing, chunk-local: every chunk of kBucketChunk consecutive
+// groups is reordered in place by size (largest first), so the order stays
+// spatially local.  No global counters.
+__global__ __launch_bounds__(kBucketThreads) void k_chunk_bucket(const int* __restrict__ count, int nb, int G,
+                                                                 int* order) {
+    __shared__ int h[kMaxGroup + 1];
+    const int lo = blockIdx.x * kBucketChunk;
+    const int hi = min(nb, lo + kBucketChunk);
+    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
+    __syncthreads();
+    int rank[kBucketPerThread], size[kBucketPerThread];
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k) {
+        const int b = lo + threadIdx.x + k * kBucketThreads;
+        size[k] = b < hi ? min(max(count[b], 1), G) : -1;
+        rank[k] = size[k] >= 0 ? atomicAdd(&h[size[k]], 1) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int n = G; n >= 1; --n) {
+            const int t = h[n];
+            h[n] = acc;
+            acc += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kBucketPerThread; ++k)
+        if (size[k] >= 1) order[lo + h[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
+}
+
 hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_ints, dim3((G + 1 + 255) / 256), dim3(256), 0, st, hist, G + 1);
     const int chunks = (nb + kBucketChunk - 1) / kBucketChunk;
     hipLaunchKernelGGL(k_size_hist, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist);
     hipLaunchKernelGGL(k_size_scan, dim3(1), dim3(64), 0, st, hist, G);
     hipLaunchKernelGGL(k_size_scatter, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_bucket_batch(const int* count, const int* starts, int nframes, int nb, int G, int mode, int* hist,
+                               int* order, hipStream_t st) {
+    if (nb <= 0) return hipSuccess;
+    const int chunks = (nb + kBucketChunk - 1) / kBucketChunk;
+    if (mode == kOrderChunk) {
+        hipLaunchKernelGGL(k_chunk_bucket, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, order);
+        return hipGetLastError();
+    }
+    const int nbins = nframes * (G + 1);
+    hipLaunchKernelGGL(k_zero_ints, dim3((nbins + 255) / 256), dim3(256), 0, st, hist, nbins);
+    hipLaunchKernelGGL(k_fsize_hist, dim3(chunks), dim3(kBucketThreads), 0, st, count, starts, nframes, nb, G, hist);
+    hipLaunchKernelGGL(k_bins_scan, dim3(1), dim3(256), 0, st, hist, nbins);
+    hipLaunchKernelGGL(k_fsize_scatter, dim3(chunks), dim3(kBucketThreads), 0, st, count, starts, nframes, nb, G, hist,
+                       order);
     return hipGetLastError();
 }
 
@@ -1178,16 +1346,16 @@ __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
     aggregate_tile(a, tile, acc, rec);
 }
 
-// Batched form: the output tiles of every frame in one persistent launch.
+// Batched form: the output tiles of every frame in one launch.
 __global__ __launch_bounds__(64) void k_aggregate_batch(const AggArgs* __restrict__ aa, const int* __restrict__ starts,
                                                         int nframes, int total) {
     __shared__ float acc[kAggLds];
     __shared__ int rec[kAggRecCap * kAggRecWords];
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-        const int f = batch_frame(starts, nframes, t);
-        aggregate_tile(aa[f], t - uniform(starts[f]), acc, rec);
-        wave_sync_lds();
-    }
+    const int t = xcd_item(total, 1, 0);
+    if (t < 0) return;
+    const int f = batch_frame(starts, nframes, t);
+    const AggArgs a = aa[f];
+    aggregate_tile(a, t - uniform(starts[f]), acc, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1381,17 +1549,24 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
     const int gm = filter8_gmax(a.p.group);
     if (a.p.block == 8 && gm > 0) {
         const size_t lds = filter8_lds_bytes(a.p.group, a.wiener);
-#define BM3D_F8(GM, W)                                                                   \
-    if (gm == GM && a.wiener == W) {                                                    \
-        hipLaunchKernelGGL((k_filter8<GM, W>), dim3(a.nb), dim3(64), lds, st, a);        \
+        const bool one = !a.process[1] && !a.process[2];
+#define BM3D_F8(GM, W, O)                                                                \
+    if (gm == GM && a.wiener == W && one == O) {                                        \
+        hipLaunchKernelGGL((k_filter8<GM, W, O>), dim3(a.nb), dim3(64), lds, st, a);     \
         return hipGetLastError();                                                        \
     }
-        BM3D_F8(8, 0)
-        BM3D_F8(16, 0)
-        BM3D_F8(32, 0)
-        BM3D_F8(8, 1)
-        BM3D_F8(16, 1)
-        BM3D_F8(32, 1)
+        BM3D_F8(8, 0, false)
+        BM3D_F8(16, 0, false)
+        BM3D_F8(32, 0, false)
+        BM3D_F8(8, 1, false)
+        BM3D_F8(16, 1, false)
+        BM3D_F8(32, 1, false)
+        BM3D_F8(8, 0, true)
+        BM3D_F8(16, 0, true)
+        BM3D_F8(32, 0, true)
+        BM3D_F8(8, 1, true)
+        BM3D_F8(16, 1, true)
+        BM3D_F8(32, 1, true)
 #undef BM3D_F8
     }
     size_t lds = filter_lds_bytes(a.p.block, a.p.group);
@@ -1423,6 +1598,13 @@ bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape&
     return ms.kmax >= 1 && ms.kmax <= 4 && ms.words * 4 * sh.match_waves <= 65536;
 }
 
+// Workgroups of a batched launch (xcd_item): one per `per` items, rounded up
+// to a multiple of 8 so that every XCD gets the same number.
+static int batch_groups(int items, int per) {
+    const long need = ((long)items + per - 1) / per;
+    return (int)((need + 7) / 8 * 8);
+}
+
 hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int nframes, int total,
                               const StageParams& p0, int max_cand, const BatchShape& sh, int* count, int* pos,
                               float* key, hipStream_t st) {
@@ -1432,9 +1614,7 @@ hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int n
     const int km = ms.kmax <= 1 ? 1 : ms.kmax <= 2 ? 2 : ms.kmax <= 3 ? 3 : 4;
     const int nw = sh.match_waves;
     const size_t lds = ms.words * 4 * nw;
-    int groups = device_cus() * (16 / nw);  // persistent: 16 waves per CU
-    const int need = (total + nw - 1) / nw;
-    groups = groups < need ? groups : need;
+    const int groups = batch_groups(total, nw);
 #define BM3D_M8B(K, WD, NW)                                                                                  \
     if (km == K && ms.w == WD && nw == NW) {                                                                 \
         hipLaunchKernelGGL((k_match8_batch<K, WD, NW>), dim3(groups), dim3(64 * NW), lds, st, frames, starts, \
@@ -1452,19 +1632,19 @@ hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int n
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_filter_batch(const FilterArgs* fa, const int* starts, int nframes, const int* order, int total,
-                               int B, int G, bool wiener, const BatchShape& sh, hipStream_t st) {
+hipError_t launch_filter_batch(const FilterArgs& a0, const BatchFilterFrame* ff, const int* starts, int nframes,
+                               const int* order, int total, const BatchShape& sh, hipStream_t st) {
     if (total <= 0) return hipSuccess;
+    const int G = a0.p.group;
     const int gm = filter8_gmax(G);
-    if (B != 8 || gm == 0) return hipErrorInvalidValue;
-    const size_t lds = filter8_lds_bytes(G, wiener);
-    int groups = device_cus() * sh.grid_per_cu;
-    groups = groups < total ? groups : total;
-#define BM3D_F8B(GM, WI)                                                                                   \
-    if (gm == GM && wiener == WI) {                                                                         \
-        hipLaunchKernelGGL((k_filter8_batch<GM, WI>), dim3(groups), dim3(64), lds, st, fa, starts, nframes, \
-                           order, total);                                                                   \
-        return hipGetLastError();                                                                           \
+    if (a0.p.block != 8 || gm == 0) return hipErrorInvalidValue;
+    const size_t lds = filter8_lds_bytes(G, a0.wiener);
+    const int groups = batch_groups(total, 1);
+#define BM3D_F8B(GM, WI)                                                                                  \
+    if (gm == GM && (a0.wiener != 0) == WI) {                                                              \
+        hipLaunchKernelGGL((k_filter8_batch<GM, WI>), dim3(groups), dim3(64), lds, st, a0, ff, starts, nframes, \
+                           order, total);                                                                  \
+        return hipGetLastError();                                                                          \
     }
     BM3D_F8B(8, false)
     BM3D_F8B(16, false)
@@ -1479,8 +1659,7 @@ hipError_t launch_filter_batch(const FilterArgs* fa, const int* starts, int nfra
 hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
                                   const BatchShape& sh, hipStream_t st) {
     if (total <= 0) return hipSuccess;
-    int groups = device_cus() * sh.grid_per_cu;
-    groups = groups < total ? groups : total;
+    const int groups = batch_groups(total, 1);
     hipLaunchKernelGGL(k_aggregate_batch, dim3(groups), dim3(64), 0, st, aa, starts, nframes, total);
     return hipGetLastError();
 }
