@@ -225,7 +225,7 @@ int enqueue_stage(bm3d_batch* b, int which) {
         HIP_TRY(launch_match(s.frames[f].plane, s.sp[f], nb, s.max_cand, cnt + s0, pos + (size_t)s0 * s.G,
                              key + (size_t)s0 * s.G, st));
         if (filter_uses_buckets(s.B, s.G))
-            HIP_TRY(launch_bucket(cnt + s0, nb, s.G, b->hist.as<int>(), b->order.as<int>() + s0, st));
+            HIP_TRY(launch_bucket(cnt + s0, nb, s.G, b->order.as<int>() + s0, st));
         FilterArgs fa = s.fa[f];
         if (!filter_uses_buckets(s.B, s.G)) fa.order = nullptr;
         HIP_TRY(launch_filter(fa, st));

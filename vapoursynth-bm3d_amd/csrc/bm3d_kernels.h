@@ -136,7 +136,7 @@ size_t filter_lds_bytes(int B, int G);
 hipError_t upload_dct8(const float* c64);
 hipError_t upload_dctz(const float* coef, int nmax);
 bool filter_fits(int B, int G, bool wiener);
-hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st);
+hipError_t launch_bucket(const int* count, int nb, int G, int* order, hipStream_t st);
 bool filter_uses_buckets(int B, int G);
 bool verify_dct_constants();
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,

@@ -405,6 +405,18 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
     }
 }
 
+// Item of consumer `sub` (a wave) of this workgroup in a launch of
+// 8 * ceil(ceil(total / per) / 8) workgroups with `per` consumers each, or -1.
+// Workgroup g runs on XCD g % 8, so the items are split into eight contiguous
+// ranges, one per XCD: neighbouring reference blocks and tiles, which read
+// overlapping pixels and contributions, share an L2.
+__device__ __forceinline__ int xcd_item(int total, int per, int sub) {
+    const int span = (total + 7) >> 3;
+    const int x = (int)(blockIdx.x & 7);
+    const int i = x * span + (int)(blockIdx.x >> 3) * per + sub;
+    return i < min(total, (x + 1) * span) ? i : -1;
+}
+
 // NW waves (reference blocks) per workgroup.  With blist, wave w of the grid
 // takes reference block blist[w] for w < *nlist (a sparse re-run).
 template <int KMAX, int W, int NW>
@@ -414,12 +426,14 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     float* __restrict__ out_key) {
     extern __shared__ uint32_t smem[];
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
-    int b = blockIdx.x * NW + wave;
+    int b;
     if (blist) {
+        b = blockIdx.x * NW + wave;
         if (b >= uniform(*nlist)) return;
         b = uniform(blist[b]);
-    } else if (b >= nb) {
-        return;
+    } else {
+        b = xcd_item(nb, NW, wave);
+        if (b < 0) return;
     }
     match8_block<KMAX, W>(plane, p, b, smem + (size_t)wave * wave_words, out_count, out_pos, out_key);
 }
@@ -445,18 +459,6 @@ __device__ __forceinline__ int lane_frame(const int* __restrict__ starts, int nf
         else hi = mid - 1;
     }
     return lo;
-}
-
-// Item of consumer `sub` (a wave) of this workgroup in a launch of
-// 8 * ceil(ceil(total / per) / 8) workgroups with `per` consumers each, or -1.
-// Workgroup g runs on XCD g % 8, so the items are split into eight contiguous
-// ranges, one per XCD: neighbouring reference blocks and tiles, which read
-// overlapping pixels and contributions, share an L2.
-__device__ __forceinline__ int xcd_item(int total, int per, int sub) {
-    const int span = (total + 7) >> 3;
-    const int x = (int)(blockIdx.x & 7);
-    const int i = x * span + (int)(blockIdx.x >> 3) * per + sub;
-    return i < min(total, (x + 1) * span) ? i : -1;
 }
 
 // Batched form: the reference blocks of every frame of a batch, numbered
@@ -1021,8 +1023,8 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
 template <int GMAX, bool WIENER, bool ONE>
 __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
-    const int slot = a.b0 + blockIdx.x;
-    if (slot >= a.nb) return;
+    const int slot = xcd_item(a.nb, 1, 0);
+    if (slot < 0) return;
     filter8_group<GMAX, WIENER, ONE>(a, a.order ? a.order[slot] : slot, fsm);
 }
 
@@ -1049,71 +1051,18 @@ __global__ __launch_bounds__(64) void k_filter8_batch(FilterArgs a0, const Batch
     filter8_group<GMAX, WIENER, true>(a, gid, fsm);
 }
 
-// Bucketing of groups by size: histogram, exclusive scan (largest n first),
-// scatter.  Counts are integers, so the buckets are deterministic; the
-// order inside a bucket does not affect any result.
+// Bucketing of groups by size, so that neighbouring waves run the same
+// group-size specialisation.  Every group writes only its own contributions,
+// so the processing order (and the order inside a bucket, which LDS atomics
+// make arbitrary) does not affect any result.
 __global__ void k_zero_ints(int* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-// Group-size bucketing.  Each workgroup histograms a chunk of the match
-// counts in LDS and touches the global counters once per non-empty size, so
-// the global atomics do not serialise on the few popular sizes.
 constexpr int kBucketThreads = 256;
 constexpr int kBucketPerThread = 16;
 constexpr int kBucketChunk = kBucketThreads * kBucketPerThread;
-
-__global__ __launch_bounds__(kBucketThreads) void k_size_hist(const int* __restrict__ count, int nb, int G,
-                                                              int* hist) {
-    __shared__ int h[kMaxGroup + 1];
-    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
-    __syncthreads();
-    const int lo = blockIdx.x * kBucketChunk;
-    const int hi = min(nb, lo + kBucketChunk);
-    for (int b = lo + threadIdx.x; b < hi; b += kBucketThreads) atomicAdd(&h[min(count[b], G)], 1);
-    __syncthreads();
-    for (int i = threadIdx.x; i <= G; i += kBucketThreads)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
-}
-
-// Exclusive prefix over sizes G..1 (largest groups first), in place.
-__global__ void k_size_scan(int* hist, int G) {
-    if (threadIdx.x != 0) return;
-    int acc = 0;
-    for (int n = G; n >= 1; --n) {
-        int c = hist[n];
-        hist[n] = acc;
-        acc += c;
-    }
-}
-
-// Places every reference block at its size's cursor: ranks inside the chunk
-// come from LDS counters, and one global atomic per size reserves the
-// chunk's range.  The order inside a size is not deterministic; it does not
-// affect any result, since every group writes only its own contributions.
-__global__ __launch_bounds__(kBucketThreads) void k_size_scatter(const int* __restrict__ count, int nb, int G,
-                                                                 int* cursor, int* order) {
-    __shared__ int h[kMaxGroup + 1];
-    __shared__ int base[kMaxGroup + 1];
-    for (int i = threadIdx.x; i <= G; i += kBucketThreads) h[i] = 0;
-    __syncthreads();
-    const int lo = blockIdx.x * kBucketChunk;
-    const int hi = min(nb, lo + kBucketChunk);
-    int rank[kBucketPerThread], size[kBucketPerThread];
-#pragma unroll
-    for (int k = 0; k < kBucketPerThread; ++k) {
-        const int b = lo + threadIdx.x + k * kBucketThreads;
-        size[k] = b < hi ? min(count[b], G) : -1;
-        rank[k] = size[k] >= 0 ? atomicAdd(&h[size[k]], 1) : 0;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i <= G; i += kBucketThreads) base[i] = h[i] ? atomicAdd(&cursor[i], h[i]) : 0;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kBucketPerThread; ++k)
-        if (size[k] >= 1) order[base[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
-}
 
 // Batched bucketing, frame-major: groups of frame 0 first, inside a frame the
 // largest sizes first.  Bin of (frame f, size n): f * (G + 1) + G - n, so an
@@ -1233,12 +1182,10 @@ __global__ __launch_bounds__(kBucketThreads) void k_chunk_bucket(const int* __re
         if (size[k] >= 1) order[lo + h[size[k]] + rank[k]] = lo + threadIdx.x + k * kBucketThreads;
 }
 
-hipError_t launch_bucket(const int* count, int nb, int G, int* hist, int* order, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_ints, dim3((G + 1 + 255) / 256), dim3(256), 0, st, hist, G + 1);
+hipError_t launch_bucket(const int* count, int nb, int G, int* order, hipStream_t st) {
+    if (nb <= 0) return hipSuccess;
     const int chunks = (nb + kBucketChunk - 1) / kBucketChunk;
-    hipLaunchKernelGGL(k_size_hist, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist);
-    hipLaunchKernelGGL(k_size_scan, dim3(1), dim3(64), 0, st, hist, G);
-    hipLaunchKernelGGL(k_size_scatter, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, hist, order);
+    hipLaunchKernelGGL(k_chunk_bucket, dim3(chunks), dim3(kBucketThreads), 0, st, count, nb, G, order);
     return hipGetLastError();
 }
 
@@ -1473,7 +1420,8 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
         const size_t wbytes = sh.words * 4;
         const int nw = wbytes * 4 <= 65536 ? 4 : wbytes * 2 <= 65536 ? 2 : wbytes <= 65536 ? 1 : 0;
         if (km != 0 && nw != 0) {
-            const int blocks = (nb + nw - 1) / nw;
+            // blist: one wave per listed block (at most nb); else xcd_item's grid
+            const int blocks = (((nb + nw - 1) / nw) + 7) / 8 * 8;
             const size_t lds = wbytes * nw;
 #define BM3D_M8(K, WD, NW)                                                                                   \
     if (km == K && sh.w == WD && nw == NW) {                                                                 \
@@ -1550,9 +1498,10 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
     if (a.p.block == 8 && gm > 0) {
         const size_t lds = filter8_lds_bytes(a.p.group, a.wiener);
         const bool one = !a.process[1] && !a.process[2];
+        const int grid = (a.nb + 7) / 8 * 8;  // xcd_item: a multiple of 8
 #define BM3D_F8(GM, W, O)                                                                \
     if (gm == GM && a.wiener == W && one == O) {                                        \
-        hipLaunchKernelGGL((k_filter8<GM, W, O>), dim3(a.nb), dim3(64), lds, st, a);     \
+        hipLaunchKernelGGL((k_filter8<GM, W, O>), dim3(grid), dim3(64), lds, st, a);     \
         return hipGetLastError();                                                        \
     }
         BM3D_F8(8, 0, false)

@@ -429,8 +429,7 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
     if (filter_uses_buckets(B, G)) {
         if ((rc = f->m_order.ensure(sizeof(int) * (size_t)nb))) return rc;
-        if ((rc = f->m_hist.ensure(sizeof(int) * (size_t)(G + 1)))) return rc;
-        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_hist.as<int>(), f->m_order.as<int>(), st));
+        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_order.as<int>(), st));
         fa.order = f->m_order.as<int>();
     }
     HIP_TRY(launch_filter(fa, st));
@@ -665,8 +664,7 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
     if (filter_uses_buckets(B, G)) {
         if ((rc = f->m_order.ensure(sizeof(int) * (size_t)nb))) return rc;
-        if ((rc = f->m_hist.ensure(sizeof(int) * (size_t)(G + 1)))) return rc;
-        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_hist.as<int>(), f->m_order.as<int>(), st));
+        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_order.as<int>(), st));
         fa.order = f->m_order.as<int>();
     }
     HIP_TRY(launch_filter(fa, st));
