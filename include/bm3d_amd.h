@@ -155,6 +155,13 @@ int bm3d_profile_read(const bm3d_filter *f, int idx, const char **name, double *
  * no counterpart in the reference's API. */
 int bm3d_debug_matches(const bm3d_filter *f, int64_t b, int *count, int *ys, int *xs, float *keys, int cap);
 
+/* The same for the last VBasic/VFinal frame, with each member's frame
+ * (index inside the frame's temporal window).  For parity tests against
+ * VBM3D_Process_Base::BlockMatching (source/VBM3D_Base.cpp:540-660); no
+ * counterpart in the reference's API. */
+int bm3d_debug_vmatches(const bm3d_filter *f, int64_t b, int *count, int *frames, int *ys, int *xs, float *keys,
+                        int cap);
+
 #ifdef __cplusplus
 }
 #endif

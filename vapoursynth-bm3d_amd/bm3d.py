@@ -72,7 +72,7 @@ class BatchShape(C.Structure):
 
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
            "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_profile",
-           "bm3d_profile_read", "bm3d_debug_matches")
+           "bm3d_profile_read", "bm3d_debug_matches", "bm3d_debug_vmatches")
 
 _lib = None
 _initialised = False
@@ -108,6 +108,8 @@ def load_library(path: str = LIB_PATH):
                                     C.POINTER(C.c_int64)]
     L.bm3d_debug_matches.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int]
+    L.bm3d_debug_vmatches.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float), C.c_int]
     # learned block-matching proposal (bm3d_lbm.h)
     L.bm3d_lbm_param_count.restype = C.c_int
     L.bm3d_lbm_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_float)]
@@ -216,6 +218,10 @@ class Clip:
 
     def clear_cache(self):
         self._cache.clear()
+
+    def evict(self, n: int) -> None:
+        """Drops cached frame n (recomputed if asked for again)."""
+        self._cache.pop(n, None)
 
     def get_frame(self, n: int) -> Frame:
         if n < 0 or n >= self.num_frames:
@@ -441,6 +447,20 @@ class _Filter:
             raise Error(self.L.bm3d_last_error().decode())
         n = cnt.value
         return np.array(ks[:n], np.float32), np.array(ys[:n], np.int32), np.array(xs[:n], np.int32)
+
+    def vmatches(self, b: int, cap: int = 512):
+        """VBasic/VFinal: (keys, frames, ys, xs) of reference block b of the last frame."""
+        cnt = C.c_int()
+        zs = (C.c_int * cap)()
+        ys = (C.c_int * cap)()
+        xs = (C.c_int * cap)()
+        ks = (C.c_float * cap)()
+        rc = self.L.bm3d_debug_vmatches(self.h, b, C.byref(cnt), zs, ys, xs, ks, cap)
+        if rc != 0:
+            raise Error(self.L.bm3d_last_error().decode())
+        n = cnt.value
+        return (np.array(ks[:n], np.float32), np.array(zs[:n], np.int32), np.array(ys[:n], np.int32),
+                np.array(xs[:n], np.int32))
 
 
     # -- learned block-matching proposal (no counterpart in the reference)

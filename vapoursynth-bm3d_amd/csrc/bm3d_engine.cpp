@@ -638,6 +638,18 @@ int bm3d_debug_matches(const bm3d_filter* f, int64_t b, int* count, int* ys, int
     return BM3D_OK;
 }
 
+int bm3d_debug_vmatches(const bm3d_filter* f, int64_t b, int* count, int* frames, int* ys, int* xs, float* keys,
+                        int cap) {
+    if (!f) return set_error(BM3D_E_STATE, "bm3d_debug_vmatches: null handle");
+    if (!f->temporal || !f->m_frame.p) return set_error(BM3D_E_ARG, "bm3d_debug_vmatches: not a VBasic/VFinal filter");
+    int rc = bm3d_debug_matches(f, b, count, ys, xs, keys, cap);
+    if (rc) return rc;
+    std::vector<int> fr(f->G);
+    HIP_TRY(hipMemcpy(fr.data(), f->m_frame.as<int>() + b * f->G, sizeof(int) * f->G, hipMemcpyDeviceToHost));
+    for (int z = 0; z < *count && z < cap; ++z) frames[z] = fr[z];
+    return BM3D_OK;
+}
+
 int bm3d_get_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst) {
     int rc = ensure_init();
     if (rc) return rc;
