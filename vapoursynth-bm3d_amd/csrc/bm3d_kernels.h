@@ -125,7 +125,7 @@ struct VAggArgs {
     float gain[3], offset[3], lo[3], hi[3];
 };
 
-size_t vmatch_lds_bytes(int max_cand, int ps_cap, int cat_cap, int B, int G);
+size_t vmatch_lds_bytes(const StageParams& p, int max_cand, int ps_cap, int cat_cap);
 hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st);
 hipError_t launch_vstack(const VStackArgs& a, hipStream_t st);
 hipError_t launch_vagg(const VAggArgs& a, hipStream_t st);

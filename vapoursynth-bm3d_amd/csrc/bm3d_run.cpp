@@ -610,7 +610,7 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     va.max_cand = side * side;
     va.ps_cap = ps_side * ps_side * std::max(1, std::min(f->ps_num, G));
     va.cat_cap = G * T;
-    if (vmatch_lds_bytes(va.max_cand, va.ps_cap, va.cat_cap, B, G) > 65536)
+    if (vmatch_lds_bytes(sp, va.max_cand, va.ps_cap, va.cat_cap) > 65536)
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the V-BM3D matching kernel");
     for (int t = 0; t < T; ++t) va.planes[t] = Rf[3 * t];
     if ((rc = f->m_count.ensure(sizeof(int) * nb))) return rc;

@@ -25,24 +25,109 @@ namespace bm3d {
 // smallest keys.  Ties are ordered by search / concatenation index.
 
 constexpr int kVMatchWaves = 2;
+constexpr int kVBitWords = 128;  // bitmap of the predictive search union (4096 positions)
 
+// Per-wave LDS layout.  STAGE: the search windows are staged in LDS (the
+// current frame's full-search window and one region per predictive seed) and
+// the SSDs read them there; otherwise they read global memory.
 struct VMatchLayout {
     int max_cand, ps_cap, cat_cap, G;
+    int rbw;        // reference block copy (B*B) or 0
+    int cw;         // current-frame window words (STAGE) or 0
+    int rw, nreg;   // words per predictive region and regions (STAGE) or 0
     __host__ __device__ int words() const {
         // keys[max(max_cand, ps_cap)] + sel (4G) + ps raw/flag/sorted
         // (3 ps_cap) + concatenated key/pos/frame (3 cat_cap) + seeds (G)
+        // + union bitmap + reference block + staged windows
         int m = max_cand > ps_cap ? max_cand : ps_cap;
-        return m + 4 * G + 3 * ps_cap + 3 * cat_cap + G;
+        return m + 4 * G + 3 * ps_cap + 3 * cat_cap + G + kVBitWords + rbw + cw + rw * nreg;
     }
 };
 
+// Window of the full search (range r) or of one predictive seed: (2r + 8)
+// rows of (2r + 8) | 1 words (odd, so lanes reading consecutive candidates
+// hit distinct banks).
+__host__ __device__ inline int vwin_stride(int r) { return (2 * r + 8) | 1; }
+__host__ __device__ inline int vwin_words(int r) { return (2 * r + 8) * vwin_stride(r); }
+
+__host__ __device__ inline VMatchLayout vmatch_layout(const StageParams& p, int max_cand, int ps_cap, int cat_cap,
+                                                      bool stage) {
+    VMatchLayout L{max_cand, ps_cap, cat_cap, p.group, 0, 0, 0, 0};
+    const int B = p.block;
+    L.rbw = (stage || B != 8) ? B * B : 0;
+    if (stage) {
+        const int range = p.range / p.step * p.step;
+        const int ps_range = p.ps_range / p.ps_step * p.ps_step;
+        L.cw = vwin_words(range);
+        L.rw = vwin_words(ps_range);
+        L.nreg = p.ps_num < p.group ? p.ps_num : p.group;
+        if (L.nreg < 1) L.nreg = 1;
+    }
+    return L;
+}
+
 // Sorted, de-duplicated union of the search windows around seeds[0..ns).
-// raw/out hold packed (y << 16) | x; returns the count (wave-uniform).
+// raw/out hold packed (y << 16) | x; returns the count (wave-uniform).  When
+// the union's bounding box fits the bitmap, positions are marked in it and
+// read back in row-major order (ascending packed value); otherwise first
+// occurrences are ranked among the distinct values.
 __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, int B, int* raw, int* flag,
-                              int* out) {
+                              int* out, uint32_t* bits) {
     const int lane = lane_id();
     const int range = p.ps_range / p.ps_step * p.ps_step;
-    const int side = 2 * (range / p.ps_step) + 1;
+    if (ns <= 0) return 0;
+    int by0 = 0x7fffffff, by1 = -1, bx0 = 0x7fffffff, bx1 = -1;
+    for (int s = 0; s < ns; ++s) {
+        const int sp = uniform(seeds[s]);
+        const int sy = sp >> 16, sx = sp & 0xffff;
+        by0 = min(by0, search_lo(sy, range, p.ps_step));
+        by1 = max(by1, search_hi(sy, p.height - B, range, p.ps_step));
+        bx0 = min(bx0, search_lo(sx, range, p.ps_step));
+        bx1 = max(bx1, search_hi(sx, p.width - B, range, p.ps_step));
+    }
+    const int bw = bx1 - bx0 + 1;
+    const int nwords = ((by1 - by0 + 1) * bw + 31) >> 5;
+    if (nwords <= kVBitWords) {
+        for (int w = lane; w < nwords; w += 64) bits[w] = 0u;
+        wave_sync_lds();
+        for (int s = 0; s < ns; ++s) {
+            const int sp = uniform(seeds[s]);
+            const int sy = sp >> 16, sx = sp & 0xffff;
+            const int l = search_lo(sx, range, p.ps_step), r = search_hi(sx, p.width - B, range, p.ps_step);
+            const int t = search_lo(sy, range, p.ps_step), b = search_hi(sy, p.height - B, range, p.ps_step);
+            const int nc = (r - l) / p.ps_step + 1, nr = (b - t) / p.ps_step + 1;
+            for (int e = lane; e < nc * nr; e += 64) {
+                const int ry = e / nc;
+                const int bit = (t + ry * p.ps_step - by0) * bw + (l + (e - ry * nc) * p.ps_step - bx0);
+                __hip_atomic_fetch_or(&bits[bit >> 5], 1u << (bit & 31), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
+            }
+        }
+        wave_sync_lds();
+        int total = 0;
+        for (int w0 = 0; w0 < nwords; w0 += 64) {
+            const int w = w0 + lane;
+            uint32_t v = w < nwords ? bits[w] : 0u;
+            const int c = __popc(v);
+            int incl = c;  // inclusive prefix over the lanes
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(incl, o);
+                if (lane >= o) incl += u;
+            }
+            int at = total + incl - c;
+            while (v) {
+                const int bb = __ffs(v) - 1;
+                v &= v - 1;
+                const int bit = (w << 5) + bb;
+                const int ry = bit / bw;
+                out[at++] = ((by0 + ry) << 16) | (bx0 + bit - ry * bw);
+            }
+            total += __shfl(incl, 63);
+        }
+        wave_sync_lds();
+        return total;
+    }
     // window of each seed (Block::AddSearchPos): row-major lattice
     int nraw = 0;
     for (int s = 0; s < ns; ++s) {
@@ -57,7 +142,6 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
         }
         nraw += nc * nr;
     }
-    (void)side;
     wave_sync_lds();
     // mark first occurrences, then place each at its rank among distinct values
     for (int e0 = 0; e0 < nraw; e0 += 64) {
@@ -89,7 +173,18 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
     return total;
 }
 
-template <int BT>
+// Copies rows t..t+nr+6, columns l..l+nc+6 of a plane (the 8x8 blocks at
+// nr x nc positions) into an LDS window of row stride ws.
+__device__ __forceinline__ void stage_window(float* win, int ws, const float* __restrict__ plane, int pitch, int t,
+                                             int l, int rows, int cols) {
+    const int lane = lane_id();
+    for (int e = lane; e < rows * cols; e += 64) {
+        const int y = e / cols, x = e - y * cols;
+        win[y * ws + x] = plane[(size_t)(t + y) * pitch + l + x];
+    }
+}
+
+template <int BT, bool STAGE>
 __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     extern __shared__ uint32_t vsm[];
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
@@ -99,8 +194,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     if (b >= a.nb) return;
     const int B = BT > 0 ? BT : p.block;
     const int G = p.group;
-    VMatchLayout L{a.max_cand, a.ps_cap, a.cat_cap, G};
-    uint32_t* base = vsm + (size_t)wave * (L.words() + (BT > 0 ? 0 : B * B));
+    const VMatchLayout L = vmatch_layout(p, a.max_cand, a.ps_cap, a.cat_cap, STAGE);
+    uint32_t* base = vsm + (size_t)wave * L.words();
     const int kcap = a.max_cand > a.ps_cap ? a.max_cand : a.ps_cap;
     uint32_t* keys = base;
     uint32_t* sel_key = keys + kcap;
@@ -112,7 +207,10 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     int* cat_pos = reinterpret_cast<int*>(cat_key + a.cat_cap);
     int* cat_frm = cat_pos + a.cat_cap;
     int* seeds = cat_frm + a.cat_cap;
-    float* rbl = reinterpret_cast<float*>(seeds + G);
+    uint32_t* bits = reinterpret_cast<uint32_t*>(seeds + G);
+    float* rbl = reinterpret_cast<float*>(bits + kVBitWords);
+    float* cwin = rbl + L.rbw;
+    float* regs = cwin + L.cw;
 
     const int bj = b / p.nx, bi = b - bj * p.nx;
     const int j = grid_pos(bj, p.block_step, p.height - B);
@@ -131,22 +229,6 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
         return;
     }
     const float* cur_plane = a.planes[a.cur];
-    float rb_reg[BT > 0 ? BT * BT : 1];
-    const float* rb = rbl;
-    if constexpr (BT > 0) {
-#pragma unroll
-        for (int e = 0; e < BT * BT; ++e) rb_reg[e] = cur_plane[(size_t)(j + e / BT) * p.pitch + i + e % BT];
-        rb = rb_reg;
-    } else {
-        for (int e = lane; e < B * B; e += 64) rbl[e] = cur_plane[(size_t)(j + e / B) * p.pitch + i + e % B];
-        wave_sync_lds();
-    }
-    auto ssd_at = [&](const float* plane, int y, int x) {
-        const float* q = plane + (size_t)y * p.pitch + x;
-        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, p.pitch);
-        else return ssd_generic(rb, q, p.pitch, B);
-    };
-
     // --- current frame: full search with the self entry first
     const int l = search_lo(i, p.range, p.step);
     const int r = search_hi(i, p.width - B, p.range, p.step);
@@ -154,6 +236,28 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     const int bt = search_hi(j, p.height - B, p.range, p.step);
     const int ncols = (r - l) / p.step + 1;
     const int nc = ncols * ((bt - t) / p.step + 1);
+    const int range = p.range / p.step * p.step;
+    const int cws = vwin_stride(range);
+    float rb_reg[(BT > 0 && !STAGE) ? BT * BT : 1];
+    const float* rb = rbl;
+    if constexpr (STAGE) {
+        stage_window(cwin, cws, cur_plane, p.pitch, t, l, bt - t + B, r - l + B);
+        wave_sync_lds();
+        for (int e = lane; e < B * B; e += 64) rbl[e] = cwin[(j - t + e / B) * cws + i - l + e % B];
+        wave_sync_lds();
+    } else if constexpr (BT > 0) {
+#pragma unroll
+        for (int e = 0; e < BT * BT; ++e) rb_reg[e] = cur_plane[(size_t)(j + e / BT) * p.pitch + i + e % BT];
+        rb = rb_reg;
+    } else {
+        for (int e = lane; e < B * B; e += 64) rbl[e] = cur_plane[(size_t)(j + e / B) * p.pitch + i + e % B];
+        wave_sync_lds();
+    }
+    auto ssd_at = [&](const float* q, int pitch) {
+        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, pitch);
+        else return ssd_generic(rb, q, pitch, B);
+    };
+
     int nacc = 0;
     for (int c0 = 0; c0 < nc; c0 += 64) {
         int c = c0 + lane;
@@ -162,7 +266,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
             int cy = c / ncols;
             int y = t + cy * p.step, x = l + (c - cy * ncols) * p.step;
             if (!(y == j && x == i)) {
-                float dist = ssd_at(cur_plane, y, x);
+                float dist = STAGE ? ssd_at(cwin + (y - t) * cws + (x - l), cws)
+                                   : ssd_at(cur_plane + (size_t)y * p.pitch + x, p.pitch);
                 if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
             }
             keys[c] = kv;
@@ -187,6 +292,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     }
     wave_sync_lds();
     const int ncur_seeds = ncat < p.ps_num ? ncat : p.ps_num;
+    const int ps_range = p.ps_range / p.ps_step * p.ps_step;
+    const int rws = vwin_stride(ps_range);
 
     // --- predictive search, backward then forward
     for (int dir = -1; dir <= 1; dir += 2) {
@@ -194,15 +301,45 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
         for (int e = lane; e < ns; e += 64) seeds[e] = cat_pos[e];
         wave_sync_lds();
         for (int f = a.cur + dir; f >= 0 && f < a.nframes; f += dir) {
-            int np_ = gen_search_pos(seeds, ns, p, B, ps_raw, ps_flag, ps_pos);
+            int np_ = gen_search_pos(seeds, ns, p, B, ps_raw, ps_flag, ps_pos, bits);
             const float* plane = a.planes[f];
+            if constexpr (STAGE) {
+                // one region per seed: the blocks of its window
+                for (int s = 0; s < ns; ++s) {
+                    const int sp = uniform(seeds[s]);
+                    const int sy = sp >> 16, sx = sp & 0xffff;
+                    const int sl = search_lo(sx, ps_range, p.ps_step);
+                    const int sr = search_hi(sx, p.width - B, ps_range, p.ps_step);
+                    const int st_ = search_lo(sy, ps_range, p.ps_step);
+                    const int sb = search_hi(sy, p.height - B, ps_range, p.ps_step);
+                    stage_window(regs + s * L.rw, rws, plane, p.pitch, st_, sl, sb - st_ + B, sr - sl + B);
+                }
+                wave_sync_lds();
+            }
             int acc = 0;
             for (int c0 = 0; c0 < np_; c0 += 64) {
                 int c = c0 + lane;
                 uint32_t kv = 0xFFFFFFFFu;
                 if (c < np_) {
                     int pk = ps_pos[c];
-                    float dist = ssd_at(plane, pk >> 16, pk & 0xffff);
+                    const int y = pk >> 16, x = pk & 0xffff;
+                    float dist;
+                    if constexpr (STAGE) {
+                        // the first seed whose window holds (y, x)
+                        int s = 0, ry = 0, rx = 0;
+                        for (; s < ns; ++s) {
+                            const int sp = seeds[s];
+                            const int sy = sp >> 16, sx = sp & 0xffff;
+                            ry = y - search_lo(sy, ps_range, p.ps_step);
+                            rx = x - search_lo(sx, ps_range, p.ps_step);
+                            if (ry >= 0 && rx >= 0 && y <= search_hi(sy, p.height - B, ps_range, p.ps_step) &&
+                                x <= search_hi(sx, p.width - B, ps_range, p.ps_step))
+                                break;
+                        }
+                        dist = ssd_at(regs + s * L.rw + ry * rws + rx, rws);
+                    } else {
+                        dist = ssd_at(plane + (size_t)y * p.pitch + x, p.pitch);
+                    }
                     if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
                     keys[c] = kv;
                 }
@@ -332,19 +469,25 @@ __global__ void k_vagg(VAggArgs a) {
 
 // ---------------------------------------------------------------------------
 
-size_t vmatch_lds_bytes(int max_cand, int ps_cap, int cat_cap, int B, int G) {
-    VMatchLayout L{max_cand, ps_cap, cat_cap, G};
-    return sizeof(uint32_t) * (size_t)kVMatchWaves * (L.words() + (B == 8 ? 0 : B * B));
+// LDS of the global-read form (the staged form is used when it fits too).
+size_t vmatch_lds_bytes(const StageParams& p, int max_cand, int ps_cap, int cat_cap) {
+    return sizeof(uint32_t) * (size_t)kVMatchWaves * vmatch_layout(p, max_cand, ps_cap, cat_cap, false).words();
 }
 
 hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st) {
-    size_t lds = vmatch_lds_bytes(a.max_cand, a.ps_cap, a.cat_cap, a.p.block, a.p.group);
-    int blocks = (a.nb + kVMatchWaves - 1) / kVMatchWaves;
+    const int blocks = (a.nb + kVMatchWaves - 1) / kVMatchWaves;
     if (blocks == 0) return hipSuccess;
+    const size_t staged =
+        sizeof(uint32_t) * (size_t)kVMatchWaves * vmatch_layout(a.p, a.max_cand, a.ps_cap, a.cat_cap, true).words();
+    if (a.p.block == 8 && staged <= 65536) {
+        hipLaunchKernelGGL((k_vmatch<8, true>), dim3(blocks), dim3(64 * kVMatchWaves), staged, st, a);
+        return hipGetLastError();
+    }
+    const size_t lds = vmatch_lds_bytes(a.p, a.max_cand, a.ps_cap, a.cat_cap);
     if (a.p.block == 8)
-        hipLaunchKernelGGL(k_vmatch<8>, dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
+        hipLaunchKernelGGL((k_vmatch<8, false>), dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
     else
-        hipLaunchKernelGGL(k_vmatch<0>, dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
+        hipLaunchKernelGGL((k_vmatch<0, false>), dim3(blocks), dim3(64 * kVMatchWaves), lds, st, a);
     return hipGetLastError();
 }
 
