@@ -76,8 +76,9 @@ LEARNED_TOPK = 32
 def learned_leg(reps=3):
     """Secondary metric: block matching of Basic(profile='fast', sigma=25,
     bm_range=24, group_size=16) on one seeded 3840x2160 Gray float frame,
-    exhaustive vs the learned proposal (top-32 re-ranked by exact SSD, trained
-    weights of lbm_train.py).  Matching time per frame from the library's
+    exhaustive vs the learned proposal (32 best even-lattice positions by
+    embedding distance, their 3x3 neighbourhoods re-ranked by exact SSD,
+    trained weights of lbm_train.py), and the same with an untrained network.  Matching time per frame from the library's
     event timers; recall of the exhaustive lists; PSNR of both Basic outputs
     against the clean image."""
     import bm3d
@@ -89,8 +90,9 @@ def learned_leg(reps=3):
     noisy, clean = synth.gray_float(2160, 3840, 25.0, seed=5999)
     clip = bm3d.Clip.from_arrays(bm3d.GRAYS, [noisy[None]])
     model = lbm.Model.load(weights)
+    untrained = lbm.Model(seed=1)
     res = {}
-    for name, m in (("exhaustive", None), ("learned", model)):
+    for name, m in (("exhaustive", None), ("learned", model), ("untrained", untrained)):
         node = bm3d.Basic(clip, **LEARNED_KW)
         if m is not None:
             node.filter.set_matcher(m, LEARNED_TOPK)
@@ -110,7 +112,12 @@ def learned_leg(reps=3):
             "speedup": round(e["ms"] / l["ms"], 3), "recall": round(lbm.recall(e["cnt"], e["pos"], l["cnt"], l["pos"]), 4),
             "psnr_exhaustive_db": round(p_e, 4), "psnr_learned_db": round(p_l, 4),
             "psnr_loss_db": round(p_e - p_l, 4), "psnr_learned_vs_exhaustive_db": round(lbm.psnr(l["out"], e["out"]), 3),
-            "fallback_blocks": l["fallback"], "ref_blocks": int(len(e["cnt"]))}
+            "fallback_blocks": l["fallback"], "ref_blocks": int(len(e["cnt"])),
+            "untrained_network": {"match_ms": round(res["untrained"]["ms"], 4),
+                                  "recall": round(lbm.recall(e["cnt"], e["pos"], res["untrained"]["cnt"],
+                                                             res["untrained"]["pos"]), 4),
+                                  "psnr_loss_db": round(p_e - lbm.psnr(res["untrained"]["out"], clean), 4),
+                                  "fallback_blocks": res["untrained"]["fallback"]}}
 
 
 V_FRAMES, V_RADIUS, V_SIGMA = 30, 3, 15.0

@@ -118,15 +118,20 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: the learned matcher supports bm_range <= 24");
     if (topk < 1 || topk > L::kMaxProposals)
         return set_error(BM3D_E_ARG, "bm3d: learned matcher top-k must be in [1, 64]");
+    if (sp.group > L::kMaxLearnedGroupSize)
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: the learned matcher supports group_size <= 64");
     const int W = sp.width, H = sp.height;
-    const size_t npos = (size_t)(W - 7) * (H - 7);
+    const size_t npos = (size_t)((W - 6) / 2) * ((H - 6) / 2);  // even lattice
     int rc;
     if ((rc = w.emb.ensure(npos * L::kEmb * sizeof(_Float16)))) return rc;
     if ((rc = w.norm.ensure(npos * sizeof(float)))) return rc;
+    if ((rc = w.remb.ensure((size_t)nb * L::kEmb * sizeof(_Float16)))) return rc;
+    if ((rc = w.rnorm.ensure((size_t)nb * sizeof(float)))) return rc;
     if ((rc = w.fallback.ensure((size_t)nb * sizeof(int)))) return rc;
     if ((rc = w.nfallback.ensure(sizeof(int)))) return rc;
     const L::Net n = m->net();
     HIP_TRY(L::launch_embed(n, plane, sp.pitch, W, H, w.emb.as<_Float16>(), w.norm.as<float>(), st));
+    HIP_TRY(L::launch_embed_refs(n, plane, sp, w.remb.as<_Float16>(), w.rnorm.as<float>(), st));
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, w.nfallback.as<int>());
     HIP_TRY(hipGetLastError());
     L::ProposeArgs pa{};
@@ -134,6 +139,8 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     pa.plane = plane;
     pa.emb = w.emb.as<_Float16>();
     pa.norm = w.norm.as<float>();
+    pa.remb = w.remb.as<_Float16>();
+    pa.rnorm = w.rnorm.as<float>();
     pa.max_cand = max_cand;
     pa.topk = topk;
     pa.count = count;

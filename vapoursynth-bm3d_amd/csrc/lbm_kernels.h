@@ -51,8 +51,10 @@ struct TrainBatch {
 struct ProposeArgs {
     StageParams p;
     const float* plane;
-    const _Float16* emb;  // [(H-7) * (W-7)][kEmb]
-    const float* norm;    // [(H-7) * (W-7)]
+    const _Float16* emb;   // even lattice: [ceil((H-7)/2) * ceil((W-7)/2)][kEmb]
+    const float* norm;     // [ceil((H-7)/2) * ceil((W-7)/2)]
+    const _Float16* remb;  // reference blocks: [nx * ny][kEmb]
+    const float* rnorm;    // [nx * ny]
     int max_cand;         // (2 range + 1)^2
     int topk;
     int* count;
@@ -73,6 +75,9 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
                            hipStream_t st);
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st);
+hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
+                             hipStream_t st);
+constexpr int kMaxLearnedGroupSize = 64;
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
 
 }  // namespace lbm

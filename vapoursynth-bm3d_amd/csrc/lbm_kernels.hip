@@ -2,9 +2,10 @@
 // 64 -> 128 (ReLU) -> 32 with BF16 weights and FP32 accumulation on MFMA
 // (v_mfma_f32_16x16x16bf16_1k), its training kernels (forward, loss,
 // backward, Adam), and the proposal path used by block matching:
-// embeddings of every patch position, FP16 embedding-distance scores of
-// every candidate in each reference block's search window, top-K proposal
-// selection, exact FP32 SSD re-rank with the plain matcher's acceptance and
+// embeddings of the even-lattice patch positions and of the reference blocks,
+// FP16 embedding-distance scores of the even lattice of each reference
+// block's search window, the best lattice positions' 3x3 neighbourhoods as
+// candidates, exact FP32 SSD re-rank with the plain matcher's acceptance and
 // ordering rules, and a list of reference blocks that fall back to the
 // exhaustive search.
 //
@@ -345,70 +346,153 @@ __global__ void k_pair_ssd(const float* __restrict__ plane, int pitch, const int
 }
 
 // ---------------------------------------------------------------------------
-// inference: embeddings of every patch position
+// inference: embeddings of the even lattice and of the reference blocks
 // ---------------------------------------------------------------------------
 
-// emb[(y * ew + x) * 32 + k] (fp16) and norm[y * ew + x] = sum_k emb^2 (fp32 of
-// the fp16 values) for y in [0, H-8], x in [0, W-8]; ew = W - 7.  Each wave
-// runs kEmbedGroups groups of 16 consecutive positions of one row.
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef short sh2 __attribute__((ext_vector_type(2)));
+
+// bf16(max(v, 0)) of two values: one packed conversion, then ReLU on the
+// packed bf16 bits as signed 16-bit integers (negative values and -0 become
+// +0), which equals rounding max(v, 0).
+__device__ __forceinline__ sh2 relu_bf16x2(float a, float b) {
+    const fl2 v = {a, b};
+    const sh2 z = {0, 0};
+    return __builtin_elementwise_max(__builtin_bit_cast(sh2, __builtin_convertvector(v, bf2)), z);
+}
+
+// The network on 16 patches (inference form of forward16: same roundings).
+__device__ __forceinline__ void infer16(const Frags& f, const s4 (&x)[4], float (&e)[2][4]) {
+    s4 hb[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f4 acc = {f.b1[t][0], f.b1[t][1], f.b1[t][2], f.b1[t][3]};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) acc = mfma(f.w1[t][s], x[s], acc);
+        const sh2 lo = relu_bf16x2(acc[0], acc[1]), hi = relu_bf16x2(acc[2], acc[3]);
+        hb[t] = s4{lo[0], lo[1], hi[0], hi[1]};
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        f4 acc = {f.b2[u][0], f.b2[u][1], f.b2[u][2], f.b2[u][3]};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) acc = mfma(f.w2[u][s], hb[s], acc);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) e[u][i] = acc[i];
+    }
+}
+
+// Patch fragments of the lane's patch at (py, px) (packed bf16 conversion).
+__device__ __forceinline__ void patch_frags(const float* __restrict__ plane, int pitch, int py, int px, s4 (&x)[4]) {
+    const int q = lane_id() >> 4;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const float* p = plane + (size_t)(py + 2 * s + (q >> 1)) * pitch + px + 4 * (q & 1);
+        float4 v;
+        __builtin_memcpy(&v, p, sizeof(v));
+        const fl2 a = {v.x, v.y}, b = {v.z, v.w};
+        const sh2 lo = __builtin_bit_cast(sh2, __builtin_convertvector(a, bf2));
+        const sh2 hi = __builtin_bit_cast(sh2, __builtin_convertvector(b, bf2));
+        x[s] = s4{lo[0], lo[1], hi[0], hi[1]};
+    }
+}
+
+// Stores the embedding (fp16) and its squared norm (fp32 of the fp16 values)
+// of the lane's patch at index idx (idx < 0: no patch).
+__device__ __forceinline__ void store_emb(const float (&e)[2][4], int idx, _Float16* __restrict__ emb,
+                                          float* __restrict__ norm) {
+    const int q = lane_id() >> 4;
+    float nrm = 0.0f;
+    h4 eh[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            eh[u][i] = (_Float16)e[u][i];
+            const float ev = (float)eh[u][i];
+            nrm += ev * ev;
+        }
+    // lanes r, r+16, r+32, r+48 hold the four quarters of one embedding
+    nrm += __shfl_xor(nrm, 16);
+    nrm += __shfl_xor(nrm, 32);
+    if (idx >= 0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) __builtin_memcpy(emb + (size_t)idx * kEmb + 16 * u + 4 * q, &eh[u], sizeof(h4));
+        if (q == 0) norm[idx] = nrm;
+    }
+}
+
+// Each wave embeds kEmbedGroups groups of 16 patches; pos(g, r, y, x)
+// returns the output index of lane r's patch in group g (and its top-left
+// corner), or -1.  The next group's patches are loaded while a group runs.
 constexpr int kEmbedGroups = 8;
-__global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ plane, int pitch, int W, int H,
-                                               _Float16* __restrict__ emb, float* __restrict__ norm) {
-    const int lane = lane_id();
-    const int ew = W - 7, eh = H - 7;
-    const int per_row = (ew + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
-    const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (wave >= per_row * eh) return;
-    const int y = wave / per_row;
-    const int xs = (wave - y * per_row) * 16 * kEmbedGroups;
+template <typename Pos>
+__device__ __forceinline__ void embed_groups(const Net& n, const float* __restrict__ plane, int pitch, Pos pos,
+                                             _Float16* __restrict__ emb, float* __restrict__ norm) {
+    const int r = lane_id() & 15;
     Frags f;
     load_frags(n, f);
-    const int r = lane & 15, q = lane >> 4;
+    int y, x;
+    int idx = pos(0, r, y, x);
     s4 nf[4];
-    {
-        float raw[4];
-        const int x = xs + r < ew ? xs + r : ew - 1;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) nf[s] = patch_frag(plane, pitch, y, x, s, raw);
-    }
+    patch_frags(plane, pitch, y, x, nf);
     for (int g = 0; g < kEmbedGroups; ++g) {
-        const int x0 = xs + 16 * g;
-        if (x0 >= ew) break;
+        if (__ballot(idx >= 0) == 0) break;
         s4 xf[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) xf[s] = nf[s];
-        if (g + 1 < kEmbedGroups && x0 + 16 < ew) {  // next group's patches in flight
-            float raw[4];
-            const int xn = x0 + 16 + r < ew ? x0 + 16 + r : ew - 1;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) nf[s] = patch_frag(plane, pitch, y, xn, s, raw);
+        const int cur = idx;
+        if (g + 1 < kEmbedGroups) {
+            idx = pos(g + 1, r, y, x);
+            patch_frags(plane, pitch, y, x, nf);
         }
-        float h[8][4], e[2][4];
-        forward16(f, xf, h, e);
-        float nrm = 0.0f;
-        _Float16 eh16[2][4];
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                eh16[u][i] = (_Float16)e[u][i];
-                const float ev = (float)eh16[u][i];
-                nrm += ev * ev;
-            }
-        // lanes r, r+16, r+32, r+48 hold the four quarters of one embedding
-        nrm += __shfl_xor(nrm, 16);
-        nrm += __shfl_xor(nrm, 32);
-        if (x0 + r < ew) {
-            const size_t base = ((size_t)y * ew + x0 + r) * kEmb;
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                _Float16* o = emb + base + 16 * u + 4 * q;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) o[i] = eh16[u][i];
-            }
-            if (q == 0) norm[(size_t)y * ew + x0 + r] = nrm;
-        }
+        float e[2][4];
+        infer16(f, xf, e);
+        store_emb(e, cur, emb, norm);
+        if (g + 1 >= kEmbedGroups) break;
     }
+}
+
+// Embeddings of the even lattice: patch (2yy, 2xx) -> index yy * ew2 + xx,
+// ew2 = ceil((W - 7) / 2), for yy < ceil((H - 7) / 2).
+__global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __restrict__ plane, int pitch, int W, int H,
+                                                       _Float16* __restrict__ emb, float* __restrict__ norm) {
+    const int ew2 = (W - 6) >> 1, eh2 = (H - 6) >> 1;
+    const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
+    const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    if (wave >= per_row * eh2) return;
+    const int yy = wave / per_row;
+    const int xs = (wave - yy * per_row) * 16 * kEmbedGroups;
+    embed_groups(
+        n, plane, pitch,
+        [&](int g, int r, int& y, int& x) {
+            const int xx = xs + 16 * g + r;
+            y = 2 * yy;
+            x = 2 * (xx < ew2 ? xx : ew2 - 1);
+            return xx < ew2 ? yy * ew2 + xx : -1;
+        },
+        emb, norm);
+}
+
+// Embeddings of the reference blocks (scan order b = by * nx + bx).
+__global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restrict__ plane, StageParams p,
+                                                    _Float16* __restrict__ emb, float* __restrict__ norm) {
+    const int nb = p.nx * p.ny;
+    const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int b0 = wave * 16 * kEmbedGroups;
+    if (b0 >= nb) return;
+    embed_groups(
+        n, plane, p.pitch,
+        [&](int g, int r, int& y, int& x) {
+            const int b = b0 + 16 * g + r;
+            const int bc = b < nb ? b : nb - 1;
+            const int by = bc / p.nx;
+            y = grid_pos(by, p.block_step, p.height - 8);
+            x = grid_pos(bc - by * p.nx, p.block_step, p.width - 8);
+            return b < nb ? b : -1;
+        },
+        emb, norm);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,32 +501,28 @@ __global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ 
 //
 // One workgroup (four waves) per tile of kPropRows x kPropCols reference
 // blocks.  Embedding distance: score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c
-// (fp32; fp16 in keys).  The dot products of the 16 blocks of a tile row with
-// 16 positions of a union row are one 16x16x32 product (two
-// v_mfma_f32_16x16x16_f16); a candidate embedding is loaded once per tile and
-// serves every tile row whose windows hold that union row.  Tiles are dealt
-// to the XCDs in column-major runs, so vertically adjacent tiles, whose
-// unions overlap most, share an L2.
+// (fp32; fp16 in the per-block lists).  The dot products of the 16 blocks of a
+// tile row with 16 lattice positions of a union row are one 16x16x32 product
+// (two v_mfma_f32_16x16x16_f16).  Tiles are dealt to the XCDs in column-major
+// runs, so vertically adjacent tiles, whose unions overlap most, share an L2.
 //
-// Pass 1 (per tile row) scores the quarter lattice of each window (every
-// other row, even columns) and takes each block's (K/2)-th best sub-score as
-// its threshold tau (about 2K full-window scores fall below it).  Pass 2
-// scores every candidate and appends the ones with score <= tau to the
-// block's list in LDS as keys (fp16 score bits << 16 | window index): unique,
-// so ordering keys orders by score with ties to the earlier candidate.
+// Scoring (per tile row): every even-lattice position (even row, even column)
+// of each block's search window is scored.  Each block proposes its M = K/2
+// best lattice positions (ties to the earlier position); the candidates are
+// the 3x3 neighbourhoods of the proposals inside the window, self excluded,
+// marked in a per-block bitmap, so they come out in window scan order.
 //
-// Per block (one wave): the K smallest keys of the list are the proposals
-// (the list must hold at least K and at most kListCap keys, else the block
-// falls back to the exhaustive search).  The proposals are re-ranked exactly
-// like the plain matcher (SSE2 lane-order SSD, dist <= thSSE && dist != 0,
-// key = float(double(dist) * distMul)); the G-1 best by (key, window index)
-// follow the self entry.  A block with fewer than G-1 accepted proposals also
-// falls back.  A window of at most K candidates is searched exhaustively here.
+// Per block (one wave): every candidate is re-ranked exactly like the plain
+// matcher (SSE2 lane-order SSD, dist <= thSSE && dist != 0, key =
+// float(double(dist) * distMul)) and the G-1 best by (key, window index)
+// follow the self entry.  A block with fewer than G-1 accepted candidates
+// falls back to the exhaustive search; a window of at most K candidates is
+// searched exhaustively here.
 constexpr int kPropCols = 16;
 constexpr int kPropRows = 4;
 constexpr int kPropBlocks = kPropRows * kPropCols;
-constexpr int kListCap = 128;
-constexpr int kSubCap = 25 * 25;  // quarter lattice of the largest window
+constexpr int kSubCap = 25 * 25;  // even lattice of the largest window
+constexpr int kListCap = 9 * (kMaxProposals / 2);
 constexpr int kBitWords = (kMaxWindow + 31) / 32;
 
 __device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
@@ -474,24 +554,41 @@ __device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restr
     return 0.0f + (((s0 + s1) + s2) + s3);
 }
 
-// k-th smallest (1-based) of the wave's values v[0..N) (0xFFFFFFFF = absent).
+// Marks v[u] (16-bit values, 0xFFFF = absent; value index lane + 64 u) that
+// are among the m smallest by (value, index).
 template <int N>
-__device__ __forceinline__ uint32_t wave_kth(const uint32_t (&v)[N], int k) {
-    uint32_t tv = 0;
-    for (int bit = 31; bit >= 0; --bit) {
+__device__ __forceinline__ uint32_t wave_smallest16(const uint32_t (&v)[N], int m) {
+    uint32_t tv = 0;  // the m-th smallest value: the largest tv with count(v < tv) < m
+    for (int bit = 15; bit >= 0; --bit) {
         const uint32_t trial = tv | (1u << bit);
         int cnt = 0;
 #pragma unroll
         for (int u = 0; u < N; ++u) cnt += __popcll(__ballot(v[u] < trial));
-        if (cnt < k) tv = trial;
+        if (cnt < m) tv = trial;
     }
-    return tv;
+    int less = 0;
+#pragma unroll
+    for (int u = 0; u < N; ++u) less += __popcll(__ballot(v[u] < tv));
+    const int need_eq = m - less;
+    int eq_seen = 0;
+    uint32_t sel = 0;
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        const bool eq = v[u] == tv && v[u] != 0xFFFFu;
+        const uint64_t em = __ballot(eq);
+        const int rank = eq_seen + __builtin_amdgcn_mbcnt_hi((uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0));
+        if ((v[u] < tv && v[u] != 0xFFFFu) || (eq && rank < need_eq)) sel |= 1u << u;
+        eq_seen += __popcll(em);
+    }
+    return sel;
 }
 
+static_assert(4 * (2 * kListCap + 4 * kMaxLearnedGroupSize) * 4 <= kPropCols * kSubCap * 2,
+              "per-wave lists must fit in the lattice score buffer");
+
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
-    __shared__ unsigned short sub[kPropCols][kSubCap];  // pass-1 scores of one tile row
-    __shared__ uint32_t bits[kPropBlocks][kBitWords];   // pass 2: candidates under tau
-    __shared__ unsigned short tau[kPropBlocks];
+    __shared__ unsigned short sub[kPropCols][kSubCap];  // lattice scores of one tile row
+    __shared__ uint32_t bits[kPropBlocks][kBitWords];   // candidates of each block
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
@@ -506,191 +603,137 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     const int tcol = lin / row_tiles;
     const int bj0 = (lin - tcol * row_tiles) * kPropRows;
     const int bi0 = tcol * kPropCols;
-    const int ew = p.width - 7;
+    const int ew2 = (p.width - 6) >> 1;
     const int B = 8;
     const int G = p.group;
     const int K = a.topk;
+    const int M = K < kMaxProposals / 2 ? K : kMaxProposals / 2;
     const int last_bi = bi0 + kPropCols - 1 < p.nx - 1 ? bi0 + kPropCols - 1 : p.nx - 1;
     const bool shortcut = G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0);
-    const int sub_rank = K / 2 > 1 ? K / 2 : 1;
-    // tile rows
-    int jr[kPropRows], tr[kPropRows], br[kPropRows];
+    int jr[kPropRows], tr[kPropRows], br[kPropRows], bjr[kPropRows];
     bool rok[kPropRows];
 #pragma unroll
     for (int m = 0; m < kPropRows; ++m) {
         rok[m] = bj0 + m < p.ny;
-        jr[m] = grid_pos(rok[m] ? bj0 + m : p.ny - 1, p.block_step, p.height - B);
+        bjr[m] = rok[m] ? bj0 + m : p.ny - 1;
+        jr[m] = grid_pos(bjr[m], p.block_step, p.height - B);
         tr[m] = search_lo(jr[m], p.range, 1);
         br[m] = search_hi(jr[m], p.height - B, p.range, 1);
     }
 
     if (!shortcut) {
         for (int e = threadIdx.x; e < kPropBlocks * kBitWords; e += 256) (&bits[0][0])[e] = 0u;
-        // columns: A row r <-> block bi0 + r; D row 4q + ii <-> block bi0 + 4q + ii
-        const int ia = grid_pos(bi0 + r <= last_bi ? bi0 + r : last_bi, p.block_step, p.width - B);
-        int lcol[4], ncol[4], icol[4];
-        uint32_t ncolv[4];
+        // A fragments: lane r <-> block bi0 + r; D rows 4q + ii <-> block bi0 + 4q + ii
+        const int bia = bi0 + r <= last_bi ? bi0 + r : last_bi;
+        int lx0[4], nlx[4];
 #pragma unroll
         for (int ii = 0; ii < 4; ++ii) {
             const int bi = bi0 + 4 * q + ii;
-            const bool ok = bi <= last_bi;
-            icol[ii] = grid_pos(ok ? bi : last_bi, p.block_step, p.width - B);
-            lcol[ii] = search_lo(icol[ii], p.range, 1);
-            ncol[ii] = search_hi(icol[ii], p.width - B, p.range, 1) - lcol[ii] + 1;
-            ncolv[ii] = ok ? (uint32_t)ncol[ii] : 0u;  // 0: no column passes the window test
+            const int ic = grid_pos(bi <= last_bi ? bi : last_bi, p.block_step, p.width - B);
+            const int l = search_lo(ic, p.range, 1), rr = search_hi(ic, p.width - B, p.range, 1);
+            lx0[ii] = (l + 1) & ~1;
+            nlx[ii] = bi <= last_bi && lx0[ii] <= rr ? ((rr - lx0[ii]) >> 1) + 1 : 0;
         }
         const int u_l = search_lo(grid_pos(bi0, p.block_step, p.width - B), p.range, 1);
         const int u_r = search_hi(grid_pos(last_bi, p.block_step, p.width - B), p.width - B, p.range, 1);
-        h4 af[kPropRows][2];
-        float nref[kPropRows][4];
-#pragma unroll
-        for (int m = 0; m < kPropRows; ++m) {
-            const _Float16* er = a.emb + ((size_t)jr[m] * ew + ia) * kEmb;
-            __builtin_memcpy(&af[m][0], er + 4 * q, sizeof(h4));
-            __builtin_memcpy(&af[m][1], er + 16 + 4 * q, sizeof(h4));
-#pragma unroll
-            for (int ii = 0; ii < 4; ++ii) nref[m][ii] = a.norm[(size_t)jr[m] * ew + icol[ii]];
-        }
-        auto load_pos = [&](int yy, int xx, h4& x0f, h4& x1f, float& nqf) {
-            const _Float16* ec = a.emb + ((size_t)yy * ew + xx) * kEmb;
-            __builtin_memcpy(&x0f, ec + 4 * q, sizeof(x0f));
-            __builtin_memcpy(&x1f, ec + 16 + 4 * q, sizeof(x1f));
-            nqf = a.norm[(size_t)yy * ew + xx];
-        };
-
-        // ---- pass 1, one tile row at a time: quarter lattice, thresholds
-        const int u_le = u_l & ~1;
-        const int ncb1 = ((u_r - u_le) / 2 + 16) >> 4;  // blocks of 16 even columns
-        const int xlast = (u_r & ~1) >= u_l ? (u_r & ~1) : u_l;
+        const int ux0 = (u_l + 1) & ~1;
+        const int nux = ux0 <= u_r ? ((u_r - ux0) >> 1) + 1 : 0;
+        const int ncb = (nux + 15) >> 4;
 #pragma unroll
         for (int m = 0; m < kPropRows; ++m) {
             if (!rok[m]) break;
+            h4 af0, af1;
+            {
+                const _Float16* er = a.remb + ((size_t)bjr[m] * p.nx + bia) * kEmb;
+                __builtin_memcpy(&af0, er + 4 * q, sizeof(h4));
+                __builtin_memcpy(&af1, er + 16 + 4 * q, sizeof(h4));
+            }
+            float nref[4];
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int bi = bi0 + 4 * q + ii;
+                nref[ii] = a.rnorm[(size_t)bjr[m] * p.nx + (bi <= last_bi ? bi : last_bi)];
+            }
             for (int e = threadIdx.x; e < kPropCols * kSubCap; e += 256) (&sub[0][0])[e] = 0xFFFFu;
             __syncthreads();
-            const int nrows = br[m] - tr[m] + 1;
-            const int nr2 = (nrows + 1) >> 1;
-            const int ntile = nr2 * ncb1;
+            const int ly0 = (tr[m] + 1) & ~1;
+            const int nly = ly0 <= br[m] ? ((br[m] - ly0) >> 1) + 1 : 0;
+            const int ntile = nly * ncb;
             for (int tl = wave; tl < ntile; tl += 4) {
-                const int mr = tl / ncb1;
-                const int x = u_le + 2 * ((tl - mr * ncb1) * 16 + r);
+                const int mr = tl / ncb;
+                const int xi = (tl - mr * ncb) * 16 + r;
+                const int x = ux0 + 2 * (xi < nux ? xi : nux - 1);
+                const int y = ly0 + 2 * mr;
+                const size_t pq = (size_t)(y >> 1) * ew2 + (x >> 1);
                 h4 b0, b1;
-                float nq;
-                load_pos(tr[m] + 2 * mr, x <= u_r ? x : xlast, b0, b1, nq);
+                __builtin_memcpy(&b0, a.emb + pq * kEmb + 4 * q, sizeof(h4));
+                __builtin_memcpy(&b1, a.emb + pq * kEmb + 16 + 4 * q, sizeof(h4));
+                const float nq = a.norm[pq];
                 f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc = mfma_h(af[m][0], b0, acc);
-                acc = mfma_h(af[m][1], b1, acc);
+                acc = mfma_h(af0, b0, acc);
+                acc = mfma_h(af1, b1, acc);
+                if (xi >= nux) continue;
 #pragma unroll
                 for (int ii = 0; ii < 4; ++ii) {
-                    const int cx = x - lcol[ii];
-                    if ((uint32_t)cx >= ncolv[ii]) continue;
-                    float d = __builtin_fmaf(-2.0f, acc[ii], nref[m][ii] + nq);
+                    const int mc = (x - lx0[ii]) >> 1;
+                    if (x < lx0[ii] || mc >= nlx[ii]) continue;
+                    float d = __builtin_fmaf(-2.0f, acc[ii], nref[ii] + nq);
                     d = d > 0.0f ? d : 0.0f;
-                    const int nc2 = (ncol[ii] + 1) >> 1;
-                    sub[4 * q + ii][mr * nc2 + (cx >> 1)] = __builtin_bit_cast(unsigned short, (_Float16)d);
+                    sub[4 * q + ii][mr * nlx[ii] + mc] = __builtin_bit_cast(unsigned short, (_Float16)d);
                 }
             }
             __syncthreads();
+            // per block: the M best lattice positions -> 3x3 neighbourhoods
             for (int k = wave; k < kPropCols; k += 4) {
                 const int bi = bi0 + k;
                 if (bi > last_bi) break;
                 const int i = grid_pos(bi, p.block_step, p.width - B);
-                const int l = search_lo(i, p.range, 1);
-                const int ncols = search_hi(i, p.width - B, p.range, 1) - l + 1;
-                const int nsub = nr2 * ((ncols + 1) >> 1);
-                // tau: the sub_rank-th smallest of the lanes' minimum sub-scores
-                // (a lane holds ~nsub/64 of them), close to the sub_rank-th
-                // smallest sub-score itself at a fraction of the cost
-                constexpr int NS2 = (kSubCap + 63) / 64;
-                uint32_t lm[1] = {0xFFFFFFFFu};
+                const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
+                const int ncols = rr - l + 1;
+                const int lxk = (l + 1) & ~1;
+                const int nlxk = lxk <= rr ? ((rr - lxk) >> 1) + 1 : 0;
+                const int nsub = nly * nlxk;
+                const int j = jr[m], t = tr[m], bt = br[m];
+                const int self = (j - t) * ncols + (i - l);
+                constexpr int NS = (kSubCap + 63) / 64;
+                uint32_t v[NS];
 #pragma unroll
-                for (int u = 0; u < NS2; ++u) {
+                for (int u = 0; u < NS; ++u) {
                     const int e = lane + 64 * u;
-                    const uint32_t v = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFFFFFu;
-                    lm[0] = v < lm[0] ? v : lm[0];
+                    v[u] = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFu;
                 }
-                const int present = __popcll(__ballot(lm[0] != 0xFFFFFFFFu));
-                const uint32_t tv = wave_kth<1>(lm, sub_rank < present ? sub_rank : present);
-                if (lane == 0) tau[m * kPropCols + k] = (unsigned short)(tv > 0xFFFFu ? 0xFFFFu : tv);
-            }
-            __syncthreads();
-        }
-
-        // ---- pass 2: every candidate of the union of the tile's windows
-        float tauf[kPropRows][4];
+                const uint32_t sel = wave_smallest16<NS>(v, M);
+                uint32_t* bk = bits[m * kPropCols + k];
 #pragma unroll
-        for (int m = 0; m < kPropRows; ++m)
+                for (int u = 0; u < NS; ++u) {
+                    if (!(sel & (1u << u))) continue;
+                    const int e = lane + 64 * u;
+                    const int my = e / nlxk;
+                    const int y0 = ly0 + 2 * my, x0 = lxk + 2 * (e - my * nlxk);
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii)
-                tauf[m][ii] = rok[m] ? (float)__builtin_bit_cast(_Float16, tau[m * kPropCols + 4 * q + ii]) : -1.0f;
-        int u_t = tr[0], u_b = br[0];
+                    for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-        for (int m = 1; m < kPropRows; ++m)
-            if (rok[m]) u_b = br[m];
-        const int ncb = (u_r - u_l + 16) >> 4;
-        const int ntile = (u_b - u_t + 1) * ncb;
-        constexpr int kPf = 4;  // tiles per group; the next group's loads are in flight
-        h4 g0[kPf], g1[kPf];
-        float gq[kPf];
-        auto load_group = [&](int tl0) {
-#pragma unroll
-            for (int u = 0; u < kPf; ++u) {
-                const int tt = tl0 + 4 * u < ntile ? tl0 + 4 * u : ntile - 1;
-                const int cy = tt / ncb;
-                const int x = u_l + (tt - cy * ncb) * 16 + r;
-                load_pos(u_t + cy, x <= u_r ? x : u_r, g0[u], g1[u], gq[u]);
-            }
-        };
-        if (wave < ntile) load_group(wave);
-        for (int tl0 = wave; tl0 < ntile; tl0 += 4 * kPf) {
-            h4 c0[kPf], c1[kPf];
-            float cq[kPf];
-#pragma unroll
-            for (int u = 0; u < kPf; ++u) {
-                c0[u] = g0[u];
-                c1[u] = g1[u];
-                cq[u] = gq[u];
-            }
-            if (tl0 + 4 * kPf < ntile) load_group(tl0 + 4 * kPf);
-#pragma unroll
-            for (int u = 0; u < kPf; ++u) {
-                const int tl = tl0 + 4 * u;
-                if (tl >= ntile) break;
-                const int cyu = tl / ncb;
-                const int y = u_t + cyu;
-                const int x = u_l + (tl - cyu * ncb) * 16 + r;
-#pragma unroll
-                for (int m = 0; m < kPropRows; ++m) {
-                    if (!rok[m] || y < tr[m] || y > br[m]) continue;
-                    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    acc = mfma_h(af[m][0], c0[u], acc);
-                    acc = mfma_h(af[m][1], c1[u], acc);
-                    uint32_t pass = 0;
-                    float dv[4];
-#pragma unroll
-                    for (int ii = 0; ii < 4; ++ii) {
-                        dv[ii] = __builtin_fmaf(-2.0f, acc[ii], nref[m][ii] + cq[u]);
-                        const bool ok = (uint32_t)(x - lcol[ii]) < ncolv[ii] && dv[ii] <= tauf[m][ii] &&
-                                        !(y == jr[m] && x == icol[ii]);
-                        pass |= ok ? (1u << ii) : 0u;
-                    }
-                    if (__ballot(pass != 0) == 0) continue;
-                    const int cy = y - tr[m];
-#pragma unroll
-                    for (int ii = 0; ii < 4; ++ii) {
-                        if (!(pass & (1u << ii))) continue;
-                        const int c = cy * ncol[ii] + (x - lcol[ii]);
-                        // no return value: the or is not waited for
-                        __hip_atomic_fetch_or(&bits[m * kPropCols + 4 * q + ii][c >> 5], 1u << (c & 31),
-                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int yy = y0 + dy, xx = x0 + dx;
+                            if (yy < t || yy > bt || xx < l || xx > rr) continue;
+                            const int c = (yy - t) * ncols + (xx - l);
+                            if (c == self) continue;
+                            __hip_atomic_fetch_or(&bk[c >> 5], 1u << (c & 31), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
                 }
             }
+            __syncthreads();  // sub is rewritten by the next tile row
         }
     }
     __syncthreads();
 
-    // ---- per block: proposals, exact re-rank, output
-    uint32_t* props_all = reinterpret_cast<uint32_t*>(&sub[0][0]);  // pass-1 scores are dead now
+    // ---- per block: candidates, exact re-rank, output
+    uint32_t* scratch = reinterpret_cast<uint32_t*>(&sub[0][0]);  // the lattice scores are dead now
+    uint32_t* cand = scratch + wave * (2 * kListCap + 4 * kMaxLearnedGroupSize);
+    uint32_t* keys = cand + kListCap;
+    uint32_t* sel_key = keys + kListCap;
+    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * kMaxLearnedGroupSize);
     for (int kb = wave; kb < kPropBlocks; kb += 4) {
         const int m = kb / kPropCols, k = kb - m * kPropCols;
         const int bi = bi0 + k;
@@ -713,134 +756,80 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
         const int ncols = rr - l + 1, nc = ncols * nrows;
         const int self = (j - t) * ncols + (i - l);
-        uint32_t* props = props_all + wave * kMaxProposals;
-        int nprop = 0;
-        bool fallback = false;
+        int n = 0;
         if (nc - 1 <= K) {
             for (int c0 = 0; c0 < nc; c0 += 64) {
                 const int c = c0 + lane;
                 const bool in = c < nc && c != self;
                 const uint64_t mk = __ballot(in);
-                const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
-                if (in) props[at] = (uint32_t)c;
-                nprop += __popcll(mk);
+                const int at = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
+                if (in) cand[at] = (uint32_t)c;
+                n += __popcll(mk);
             }
         } else {
-            // candidates under tau: the set bits of the block's bitmap
+            // the set bits of the block's bitmap, in window order
             const int nw = (nc + 31) >> 5;
-            uint32_t w0 = lane < nw ? bits[kb][lane] : 0u;
-            uint32_t w1 = lane + 64 < nw ? bits[kb][lane + 64] : 0u;
+            const uint32_t w0 = lane < nw ? bits[kb][lane] : 0u;
+            const uint32_t w1 = lane + 64 < nw ? bits[kb][lane + 64] : 0u;
             const int c0n = __popc(w0), c1n = __popc(w1);
-            // exclusive prefix of the per-lane counts (word order = candidate order)
-            int incl = c0n;
+            int incl = c0n, incl1 = c1n;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
+                const int v0 = __shfl_up(incl, o), v1 = __shfl_up(incl1, o);
+                if (lane >= o) {
+                    incl += v0;
+                    incl1 += v1;
+                }
             }
             const int tot0 = __shfl(incl, 63);
-            int incl1 = c1n;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(incl1, o);
-                if (lane >= o) incl1 += v;
-            }
-            const int n = tot0 + __shfl(incl1, 63);
-            if (n < K || n > kListCap) {
-                fallback = true;
-            } else {
-                uint32_t* cand = props_all + 4 * kMaxProposals + wave * kListCap;
-                int at = incl - c0n;
-                for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
-                at = tot0 + incl1 - c1n;
-                for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
-                wave_sync_lds();
-                // scores of the listed candidates (fp16 bits as in pass 2) -> keys
-                const _Float16* er = a.emb + ((size_t)j * ew + i) * kEmb;
-                uint32_t ew32[16];
-                __builtin_memcpy(ew32, er, sizeof(ew32));
-                const float nr = a.norm[(size_t)j * ew + i];
-                constexpr int SR = kListCap / 64;
-                uint32_t ck[SR];
-#pragma unroll
-                for (int u = 0; u < SR; ++u) {
-                    const int e = lane + 64 * u;
-                    ck[u] = 0xFFFFFFFFu;
-                    if (e < n) {
-                        const int c = (int)cand[e];
-                        const int cy = c / ncols, cx = c - cy * ncols;
-                        const size_t pq = (size_t)(t + cy) * ew + l + cx;
-                        uint32_t eq[16];
-                        __builtin_memcpy(eq, a.emb + pq * kEmb, sizeof(eq));
-                        float dot = 0.0f;
-#pragma unroll
-                        for (int w = 0; w < 16; ++w)
-                            dot = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, ew32[w]), __builtin_bit_cast(h2, eq[w]),
-                                                         dot, false);
-                        float d = __builtin_fmaf(-2.0f, dot, nr + a.norm[pq]);
-                        d = d > 0.0f ? d : 0.0f;
-                        ck[u] = ((uint32_t)__builtin_bit_cast(unsigned short, (_Float16)d) << 16) | (uint32_t)c;
-                    }
-                }
-                const uint32_t tv = wave_kth<SR>(ck, K);
-#pragma unroll
-                for (int u = 0; u < SR; ++u) {
-                    const bool in = ck[u] <= tv;
-                    const uint64_t mk = __ballot(in);
-                    const int at = nprop + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
-                    if (in) props[at] = ck[u] & 0xffffu;
-                    nprop += __popcll(mk);
-                }
-            }
-        }
-        if (fallback) {
-            if (lane == 0) {
-                const int at = atomicAdd(a.nfallback, 1);
-                a.fallback[at] = b;
-            }
-            continue;
+            n = tot0 + __shfl(incl1, 63);
+            int at = incl - c0n;
+            for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
+            at = tot0 + incl1 - c1n;
+            for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
         }
         wave_sync_lds();
-        // ---- exact re-rank, one lane per proposal (nprop <= 64)
+        // exact SSD of every candidate, one lane per candidate
         float rb[64];
 #pragma unroll
         for (int e = 0; e < 64; ++e) rb[e] = a.plane[(size_t)(j + e / 8) * p.pitch + i + e % 8];
-        uint32_t kv = 0xFFFFFFFFu;
-        int c = 0x7FFFFFFF;
-        if (lane < nprop) {
-            c = (int)props[lane];
-            const int cy = c / ncols, cx = c - cy * ncols;
-            const float dist = ssd8_rows(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
-            if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+        int nacc = 0;
+        for (int e0 = 0; e0 < n; e0 += 64) {
+            const int e = e0 + lane;
+            uint32_t kv = 0xFFFFFFFFu;
+            if (e < n) {
+                const int c = (int)cand[e];
+                const int cy = c / ncols, cx = c - cy * ncols;
+                const float dist = ssd8_rows(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
+                if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+                keys[e] = kv;
+            }
+            nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
         }
-        wave_sync_lds();  // props is rewritten by this wave's next block
-        const int nacc = __popcll(__ballot(kv != 0xFFFFFFFFu));
-        const int kk = nacc < G - 1 ? nacc : G - 1;
+        wave_sync_lds();
         if (nacc < G - 1 && nc - 1 > K) {
-            // not enough acceptable proposals: the exhaustive search decides
+            // not enough acceptable candidates: the exhaustive search decides
             if (lane == 0) {
                 const int at = atomicAdd(a.nfallback, 1);
                 a.fallback[at] = b;
             }
             continue;
         }
-        // rank by (key, window index) against every other proposal
-        int rank = 0;
-        for (int o = 0; o < nprop; ++o) {
-            const uint32_t k2 = __builtin_amdgcn_readlane(kv, o);
-            const int c2 = __builtin_amdgcn_readlane(c, o);
-            rank += (k2 < kv) || (k2 == kv && c2 < c);
-        }
-        if (kv != 0xFFFFFFFFu && rank < kk) {
+        const int kk = nacc < G - 1 ? nacc : G - 1;
+        // the kk best by (key, list position); list position order = window order
+        wave_select(keys, n, nacc, kk, sel_key, sel_idx);
+        for (int z = lane; z < kk; z += 64) {
+            const int c = (int)cand[sel_idx[kk + z]];
             const int cy = c / ncols;
-            pos_out[1 + rank] = ((t + cy) << 16) | (l + c - cy * ncols);
-            key_out[1 + rank] = __uint_as_float(kv);
+            pos_out[1 + z] = ((t + cy) << 16) | (l + c - cy * ncols);
+            key_out[1 + z] = __uint_as_float(sel_key[kk + z]);
         }
         if (lane == 0) {
             *cnt_out = kk + 1;
             pos_out[0] = (j << 16) | i;
             key_out[0] = 0.0f;
         }
+        wave_sync_lds();  // cand / keys are rewritten for this wave's next block
     }
 }
 
@@ -890,11 +879,20 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
 
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st) {
-    const int ew = W - 7, eh = H - 7;
-    if (ew <= 0 || eh <= 0) return hipSuccess;
-    const int per_row = (ew + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
-    const int waves = per_row * eh;
-    hipLaunchKernelGGL(k_embed, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, pitch, W, H, emb, norm);
+    const int ew2 = (W - 6) >> 1, eh2 = (H - 6) >> 1;
+    if (ew2 <= 0 || eh2 <= 0) return hipSuccess;
+    const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
+    const int waves = per_row * eh2;
+    hipLaunchKernelGGL(k_embed_lattice, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, pitch, W, H, emb, norm);
+    return hipGetLastError();
+}
+
+hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
+                             hipStream_t st) {
+    const int nb = p.nx * p.ny;
+    if (nb <= 0) return hipSuccess;
+    const int waves = (nb + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
+    hipLaunchKernelGGL(k_embed_refs, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, p, emb, norm);
     return hipGetLastError();
 }
 
