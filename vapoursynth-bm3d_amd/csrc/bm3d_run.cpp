@@ -179,6 +179,23 @@ void opp2rgb_coefs(int src_bits, int dst_bits, float m[9], float off[3], float& 
 
 size_t align_pitch(int w) { return (size_t)((w + kPitchAlign - 1) / kPitchAlign) * kPitchAlign; }
 
+// Float planes already in device memory are read in place when every plane a
+// call needs shares one row stride: returns that pitch (elements), else 0.
+size_t direct_float_pitch(const bm3d_clip_info& vi, const bm3d_frame* const* frames, int nframes, const bool need[3]) {
+    if (vi.sample_type != BM3D_ST_FLOAT || vi.color_family == BM3D_CF_RGB) return 0;
+    ptrdiff_t stride = 0;
+    for (int t = 0; t < nframes; ++t)
+        for (int p = 0; p < 3; ++p) {
+            if (!need[p]) continue;
+            const bm3d_frame& fr = *frames[t];
+            if (fr.mem != 1 || fr.stride[p] <= 0 || fr.stride[p] % (ptrdiff_t)sizeof(float) != 0) return 0;
+            if (stride == 0) stride = fr.stride[p];
+            if (fr.stride[p] != stride) return 0;
+        }
+    if (stride == 0 || (size_t)stride / sizeof(float) < (size_t)vi.width) return 0;
+    return (size_t)stride / sizeof(float);
+}
+
 // Returns a device pointer + element pitch for plane p of frame fr, staging
 // host memory into buf when needed.
 int device_plane(const bm3d_frame& fr, int p, int w, int h, int bps, DevBuf& buf, hipStream_t st,
@@ -300,7 +317,7 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
         log_warning(std::string(f->wiener ? "bm3d.Basic/bm3d.Final" : "bm3d.Basic/bm3d.Final") +
                     " - warning: There's a frame property \"BM3D_OPP=1\" indicating opponent color space input. "
                     "You should specify \"matrix=100\" in the filter's argument.");
-    const size_t P_ = align_pitch(W);
+    size_t P_ = align_pitch(W);
     const hipStream_t st = f->stream;
     int rc;
     int proc[3] = {f->process[0], gray_path ? 0 : f->process[1], gray_path ? 0 : f->process[2]};
@@ -344,12 +361,28 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
         if (!f->rdef)
             for (int p = 0; p < 3; ++p) Rf[p] = S[p];
     } else {
-        for (int which = 0; which < (f->rdef ? 2 : 1); ++which) {
+        bool need_w[2][3];
+        for (int p = 0; p < 3; ++p) {
+            need_w[0][p] = p < P && (proc[p] || (p == 0 && !f->rdef));
+            need_w[1][p] = p < P && (p == 0 || (f->wiener && proc[p]));
+        }
+        // float planes already in device memory with one common stride are read in place
+        const bm3d_frame* fs = &s;
+        const bm3d_frame* fr0 = &rf;
+        const size_t ps = direct_float_pitch(vi, &fs, 1, need_w[0]);
+        const size_t pr = f->rdef ? direct_float_pitch(vi, &fr0, 1, need_w[1]) : ps;
+        const bool direct = ps != 0 && ps == pr;
+        if (direct) {
+            P_ = ps;
+            for (int p = 0; p < 3; ++p) {
+                if (need_w[0][p]) S[p] = static_cast<const float*>(s.data[p]);
+                if (f->rdef && need_w[1][p]) Rf[p] = static_cast<const float*>(rf.data[p]);
+            }
+        }
+        for (int which = 0; which < (f->rdef ? 2 : 1) && !direct; ++which) {
             const bm3d_frame& fr = which ? rf : s;
             for (int p = 0; p < P; ++p) {
-                bool need;
-                if (which == 0) need = proc[p] || (p == 0 && !f->rdef);
-                else need = p == 0 || (f->wiener && proc[p]);
+                const bool need = need_w[which][p];
                 if (!need) continue;
                 const void* ptr;
                 size_t pe;
@@ -572,16 +605,44 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     int proc[3] = {f->process[0], gray_path ? 0 : f->process[1], gray_path ? 0 : f->process[2]};
     if (P == 1) proc[1] = proc[2] = 0;
     if (rgb) proc[0] = proc[1] = proc[2] = 1;
-    const size_t P_ = align_pitch(W);
+    size_t P_ = align_pitch(W);
     const hipStream_t st = f->stream;
     int rc;
     // --- float planes of every frame of the window
     std::vector<const float*> S(3 * T, nullptr), Rf(3 * T, nullptr);
-    for (int t = 0; t < T; ++t) {
+    bool need_s0[3], need_r0[3];
+    for (int p = 0; p < 3; ++p) {
+        need_s0[p] = p < P && (proc[p] || (p == 0 && !f->rdef));
+        need_r0[p] = p < P && (p == 0 || (f->wiener && proc[p]));
+    }
+    size_t direct = 0;  // float device planes read in place
+    {
+        std::vector<const bm3d_frame*> fs(T), fr(T);
+        for (int t = 0; t < T; ++t) {
+            fs[t] = &src[t];
+            fr[t] = f->rdef ? &ref[t] : &src[t];
+        }
+        const size_t ps = direct_float_pitch(vi, fs.data(), T, need_s0);
+        const size_t pr = f->rdef ? direct_float_pitch(vi, fr.data(), T, need_r0) : ps;
+        if (ps != 0 && ps == pr) direct = ps;
+    }
+    if (direct) {
+        P_ = direct;
+        for (int t = 0; t < T; ++t)
+            for (int p = 0; p < 3; ++p) {
+                if (need_s0[p]) S[3 * t + p] = static_cast<const float*>(src[t].data[p]);
+                if (f->rdef) {
+                    if (need_r0[p]) Rf[3 * t + p] = static_cast<const float*>(ref[t].data[p]);
+                } else {
+                    Rf[3 * t + p] = S[3 * t + p];
+                }
+            }
+    }
+    for (int t = 0; t < T && !direct; ++t) {
         bool need_s[3], need_r[3];
         for (int p = 0; p < 3; ++p) {
-            need_s[p] = p < P && (proc[p] || (p == 0 && !f->rdef));
-            need_r[p] = p < P && (p == 0 || (f->wiener && proc[p]));
+            need_s[p] = need_s0[p];
+            need_r[p] = need_r0[p];
         }
         if (rgb) need_s[0] = need_s[1] = need_s[2] = true;
         const float* so[3] = {nullptr, nullptr, nullptr};
