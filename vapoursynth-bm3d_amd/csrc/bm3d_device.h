@@ -226,6 +226,67 @@ __device__ inline void wave_select(const uint32_t* keys, int nc, int naccepted, 
     rank_sort_selected(sel_key, sel_idx, k);
 }
 
+// wave_select for at most 64 * NR candidates, with the keys held in
+// registers during the radix descent (one LDS read per key instead of one
+// per bit).  Same result as wave_select.
+template <int NR>
+__device__ inline void wave_select_small(const uint32_t* keys, int nc, int naccepted, int k, uint32_t* sel_key,
+                                         int* sel_idx) {
+    const int lane = lane_id();
+    uint32_t v[NR];
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int c = lane + 64 * u;
+        v[u] = c < nc ? keys[c] : 0xFFFFFFFFu;
+    }
+    uint32_t thr = 0xFFFFFFFEu;
+    int need_eq = 0;
+    const bool cut = naccepted > k;
+    if (cut) {
+        uint32_t t = 0;
+        for (int bit = 31; bit >= 0; --bit) {
+            const uint32_t trial = t | (1u << bit);
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < NR; ++u) cnt += __popcll(__ballot(v[u] < trial));
+            if (cnt < k) t = trial;
+        }
+        int less_cnt = 0;
+#pragma unroll
+        for (int u = 0; u < NR; ++u) less_cnt += __popcll(__ballot(v[u] < t));
+        thr = t;
+        need_eq = k - less_cnt;
+    }
+    int base = 0, eq_seen = 0;
+#pragma unroll
+    for (int u = 0; u < NR; ++u) {
+        const int c = lane + 64 * u;
+        const uint32_t kv = v[u];
+        bool sel;
+        if (cut) {
+            const bool eq = kv == thr;
+            const uint64_t eqm = __ballot(eq);
+            const int eq_rank = eq_seen + __builtin_amdgcn_mbcnt_hi((uint32_t)(eqm >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)eqm, 0));
+            sel = (kv < thr) || (eq && eq_rank < need_eq);
+            eq_seen += __popcll(eqm);
+        } else {
+            sel = kv != 0xFFFFFFFFu;
+        }
+        const uint64_t m = __ballot(sel);
+        const int pos = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        if (sel) {
+            sel_key[pos] = kv;
+            sel_idx[pos] = c;
+        }
+        base += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    rank_sort_selected(sel_key, sel_idx, k);
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -104,26 +104,28 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
             }
         }
         wave_sync_lds();
+        // positions in bit order, 64 bits (two words) per pass: lane l tests bit
+        // b0 + l; the row is tracked incrementally (no division)
+        const int nbits = (by1 - by0 + 1) * bw;
         int total = 0;
-        for (int w0 = 0; w0 < nwords; w0 += 64) {
-            const int w = w0 + lane;
-            uint32_t v = w < nwords ? bits[w] : 0u;
-            const int c = __popc(v);
-            int incl = c;  // inclusive prefix over the lanes
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int u = __shfl_up(incl, o);
-                if (lane >= o) incl += u;
+        int ry = lane / bw, rx = lane - (lane / bw) * bw;  // row / column of bit `lane`
+        const int step_y = 64 / bw, step_x = 64 - (64 / bw) * bw;
+        for (int b0 = 0; b0 < nbits; b0 += 64) {
+            const int bit = b0 + lane;
+            const bool set = bit < nbits && ((bits[bit >> 5] >> (bit & 31)) & 1u);
+            const uint64_t m = __ballot(set);
+            if (set) {
+                const int at = total + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                out[at] = ((by0 + ry) << 16) | (bx0 + rx);
             }
-            int at = total + incl - c;
-            while (v) {
-                const int bb = __ffs(v) - 1;
-                v &= v - 1;
-                const int bit = (w << 5) + bb;
-                const int ry = bit / bw;
-                out[at++] = ((by0 + ry) << 16) | (bx0 + bit - ry * bw);
+            total += __popcll(m);
+            // advance this lane's bit by 64 = step_y rows and step_x columns
+            ry += step_y;
+            rx += step_x;
+            if (rx >= bw) {
+                rx -= bw;
+                ++ry;
             }
-            total += __shfl(incl, 63);
         }
         wave_sync_lds();
         return total;
@@ -276,7 +278,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     }
     wave_sync_lds();
     int k = nacc < G - 1 ? nacc : G - 1;
-    wave_select(keys, nc, nacc, k, sel_key, sel_idx);
+    if (nc <= 256) wave_select_small<4>(keys, nc, nacc, k, sel_key, sel_idx);
+    else wave_select(keys, nc, nacc, k, sel_key, sel_idx);
     int ncat = 1 + k;
     if (lane == 0) {
         cat_key[0] = 0u;
@@ -347,7 +350,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
             }
             wave_sync_lds();
             int kf = acc < G ? acc : G;
-            wave_select(keys, np_, acc, kf, sel_key, sel_idx);
+            if (np_ <= 256) wave_select_small<4>(keys, np_, acc, kf, sel_key, sel_idx);
+            else wave_select(keys, np_, acc, kf, sel_key, sel_idx);
             for (int e = lane; e < kf; e += 64) {
                 cat_key[ncat + e] = sel_key[kf + e];
                 cat_pos[ncat + e] = ps_pos[sel_idx[kf + e]];
