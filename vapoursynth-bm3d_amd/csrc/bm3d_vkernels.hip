@@ -180,9 +180,15 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
 __device__ __forceinline__ void stage_window(float* win, int ws, const float* __restrict__ plane, int pitch, int t,
                                              int l, int rows, int cols) {
     const int lane = lane_id();
-    for (int e = lane; e < rows * cols; e += 64) {
-        const int y = e / cols, x = e - y * cols;
-        win[y * ws + x] = plane[(size_t)(t + y) * pitch + l + x];
+    if (cols <= 64) {
+        // 64 / cols rows per pass, one division per call
+        const int rpp = 64 / cols;
+        const int yo = lane / cols, x = lane - yo * cols;
+        if (yo < rpp)
+            for (int y = yo; y < rows; y += rpp) win[y * ws + x] = plane[(size_t)(t + y) * pitch + l + x];
+    } else {
+        for (int y = 0; y < rows; ++y)
+            for (int x = lane; x < cols; x += 64) win[y * ws + x] = plane[(size_t)(t + y) * pitch + l + x];
     }
 }
 
