@@ -817,7 +817,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         }
         const int kk = nacc < G - 1 ? nacc : G - 1;
         // the kk best by (key, list position); list position order = window order
-        wave_select(keys, n, nacc, kk, sel_key, sel_idx);
+        wave_select_small<(kListCap + 63) / 64>(keys, n, nacc, kk, sel_key, sel_idx);
         for (int z = lane; z < kk; z += 64) {
             const int c = (int)cand[sel_idx[kk + z]];
             const int cy = c / ncols;
