@@ -659,16 +659,27 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             const int ly0 = (tr[m] + 1) & ~1;
             const int nly = ly0 <= br[m] ? ((br[m] - ly0) >> 1) + 1 : 0;
             const int ntile = nly * ncb;
+            // one step = 16 lattice positions of one lattice row; the next step's
+            // embeddings are loaded while this one is scored
+            auto load_step = [&](int tl, h4& b0, h4& b1, float& nq) {
+                const int mr = tl / ncb;
+                const int xi = (tl - mr * ncb) * 16 + r;
+                const int x = ux0 + 2 * (xi < nux ? xi : nux - 1);
+                const size_t pq = (size_t)((ly0 >> 1) + mr) * ew2 + (x >> 1);
+                __builtin_memcpy(&b0, a.emb + pq * kEmb + 4 * q, sizeof(h4));
+                __builtin_memcpy(&b1, a.emb + pq * kEmb + 16 + 4 * q, sizeof(h4));
+                nq = a.norm[pq];
+            };
+            h4 n0, n1;
+            float nnq = 0.0f;
+            if (wave < ntile) load_step(wave, n0, n1, nnq);
             for (int tl = wave; tl < ntile; tl += 4) {
                 const int mr = tl / ncb;
                 const int xi = (tl - mr * ncb) * 16 + r;
                 const int x = ux0 + 2 * (xi < nux ? xi : nux - 1);
-                const int y = ly0 + 2 * mr;
-                const size_t pq = (size_t)(y >> 1) * ew2 + (x >> 1);
-                h4 b0, b1;
-                __builtin_memcpy(&b0, a.emb + pq * kEmb + 4 * q, sizeof(h4));
-                __builtin_memcpy(&b1, a.emb + pq * kEmb + 16 + 4 * q, sizeof(h4));
-                const float nq = a.norm[pq];
+                const h4 b0 = n0, b1 = n1;
+                const float nq = nnq;
+                if (tl + 4 < ntile) load_step(tl + 4, n0, n1, nnq);
                 f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 acc = mfma_h(af0, b0, acc);
                 acc = mfma_h(af1, b1, acc);
