@@ -38,9 +38,12 @@ struct VMatchLayout {
     __host__ __device__ int words() const {
         // keys[max(max_cand, ps_cap)] + sel (4G) + ps raw/flag/sorted
         // (3 ps_cap) + concatenated key/pos/frame (3 cat_cap) + seeds (G)
-        // + union bitmap + reference block + staged windows
+        // + union bitmap + reference block + staged windows (the current
+        // frame's window and the predictive regions share one buffer: the
+        // regions are staged only after the current frame is searched)
         int m = max_cand > ps_cap ? max_cand : ps_cap;
-        return m + 4 * G + 3 * ps_cap + 3 * cat_cap + G + kVBitWords + rbw + cw + rw * nreg;
+        const int stage = cw > rw * nreg ? cw : rw * nreg;
+        return m + 4 * G + 3 * ps_cap + 3 * cat_cap + G + kVBitWords + rbw + stage;
     }
 };
 
@@ -96,12 +99,17 @@ __device__ int gen_search_pos(const int* seeds, int ns, const StageParams& p, in
             const int l = search_lo(sx, range, p.ps_step), r = search_hi(sx, p.width - B, range, p.ps_step);
             const int t = search_lo(sy, range, p.ps_step), b = search_hi(sy, p.height - B, range, p.ps_step);
             const int nc = (r - l) / p.ps_step + 1, nr = (b - t) / p.ps_step + 1;
-            for (int e = lane; e < nc * nr; e += 64) {
-                const int ry = e / nc;
-                const int bit = (t + ry * p.ps_step - by0) * bw + (l + (e - ry * nc) * p.ps_step - bx0);
-                __hip_atomic_fetch_or(&bits[bit >> 5], 1u << (bit & 31), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WAVEFRONT);
-            }
+            // 64 / nc window rows per pass (one division per seed)
+            const int rpp = nc <= 64 ? 64 / nc : 1;
+            const int yo = nc <= 64 ? lane / nc : 0;
+            const int xo = nc <= 64 ? lane - yo * nc : lane;
+            if (yo < rpp)
+                for (int ry = yo; ry < nr; ry += rpp)
+                    for (int rx = xo; rx < nc; rx += (nc <= 64 ? nc : 64)) {
+                        const int bit = (t + ry * p.ps_step - by0) * bw + (l + rx * p.ps_step - bx0);
+                        __hip_atomic_fetch_or(&bits[bit >> 5], 1u << (bit & 31), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_WAVEFRONT);
+                    }
         }
         wave_sync_lds();
         // positions in bit order, 64 bits (two words) per pass: lane l tests bit
@@ -218,7 +226,7 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     uint32_t* bits = reinterpret_cast<uint32_t*>(seeds + G);
     float* rbl = reinterpret_cast<float*>(bits + kVBitWords);
     float* cwin = rbl + L.rbw;
-    float* regs = cwin + L.cw;
+    float* regs = cwin;  // the current-frame window is dead once the predictive search starts
 
     const int bj = b / p.nx, bi = b - bj * p.nx;
     const int j = grid_pos(bj, p.block_step, p.height - B);
