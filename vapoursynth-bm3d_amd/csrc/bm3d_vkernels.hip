@@ -292,7 +292,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     }
     wave_sync_lds();
     int k = nacc < G - 1 ? nacc : G - 1;
-    if (nc <= 256) wave_select_small<4>(keys, nc, nacc, k, sel_key, sel_idx);
+    if (nc <= 128) wave_select_small<2>(keys, nc, nacc, k, sel_key, sel_idx);
+    else if (nc <= 256) wave_select_small<4>(keys, nc, nacc, k, sel_key, sel_idx);
     else wave_select(keys, nc, nacc, k, sel_key, sel_idx);
     int ncat = 1 + k;
     if (lane == 0) {
@@ -364,7 +365,10 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
             }
             wave_sync_lds();
             int kf = acc < G ? acc : G;
-            if (np_ <= 256) wave_select_small<4>(keys, np_, acc, kf, sel_key, sel_idx);
+            if (np_ <= 64) wave_select_small<1>(keys, np_, acc, kf, sel_key, sel_idx);
+            else if (np_ <= 128) wave_select_small<2>(keys, np_, acc, kf, sel_key, sel_idx);
+            else if (np_ <= 192) wave_select_small<3>(keys, np_, acc, kf, sel_key, sel_idx);
+            else if (np_ <= 256) wave_select_small<4>(keys, np_, acc, kf, sel_key, sel_idx);
             else wave_select(keys, np_, acc, kf, sel_key, sel_idx);
             for (int e = lane; e < kf; e += 64) {
                 cat_key[ncat + e] = sel_key[kf + e];
@@ -392,7 +396,9 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
     }
     // select G-1 of entries 1..ncat-1 by (key, concatenation index)
     int m = ncat - 1;
-    wave_select(cat_key + 1, m, m, G - 1, sel_key, sel_idx);
+    if (m <= 64) wave_select_small<1>(cat_key + 1, m, m, G - 1, sel_key, sel_idx);
+    else if (m <= 256) wave_select_small<4>(cat_key + 1, m, m, G - 1, sel_key, sel_idx);
+    else wave_select(cat_key + 1, m, m, G - 1, sel_key, sel_idx);
     if (lane == 0) {
         *cnt_out = G;
         pos_out[0] = cat_pos[0];
