@@ -121,7 +121,7 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     if (sp.group > L::kMaxLearnedGroupSize)
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: the learned matcher supports group_size <= 64");
     const int W = sp.width, H = sp.height;
-    const size_t npos = (size_t)((W - 6) / 2) * ((H - 6) / 2);  // even lattice
+    const size_t npos = (size_t)((W - 7 + L::kLat - 1) / L::kLat) * ((H - 7 + L::kLat - 1) / L::kLat);  // lattice
     int rc;
     if ((rc = w.emb.ensure(npos * L::kEmb * sizeof(_Float16)))) return rc;
     if ((rc = w.norm.ensure(npos * sizeof(float)))) return rc;

@@ -20,6 +20,7 @@ constexpr int kOffW2 = kOffB1 + kHidden;
 constexpr int kOffB2 = kOffW2 + kEmb * kHidden;
 constexpr int kParams = kOffB2 + kEmb;
 constexpr int kMaxProposals = 64;   // largest top-K
+constexpr int kLat = 2;             // proposal lattice stride (rows and columns)
 constexpr int kMaxWindow = 49 * 49; // largest search window (bm_range 24)
 
 // Device view of one network: FP32 master parameters and the BF16 copies of

@@ -349,6 +349,13 @@ __global__ void k_pair_ssd(const float* __restrict__ plane, int pitch, const int
 // inference: embeddings of the even lattice and of the reference blocks
 // ---------------------------------------------------------------------------
 
+// The proposal lattice: every kLat-th row and column.  lat_count(n): lattice
+// points among positions 0..n-1; lat_first(v): the first lattice coordinate
+// >= v; lat_span(f, b): lattice points from f (a lattice coordinate) to b.
+__host__ __device__ constexpr int lat_count(int n) { return n > 0 ? (n + kLat - 1) / kLat : 0; }
+__host__ __device__ constexpr int lat_first(int v) { return (v + kLat - 1) / kLat * kLat; }
+__host__ __device__ constexpr int lat_span(int f, int b) { return f <= b ? (b - f) / kLat + 1 : 0; }
+
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 typedef float fl2 __attribute__((ext_vector_type(2)));
 typedef short sh2 __attribute__((ext_vector_type(2)));
@@ -454,11 +461,11 @@ __device__ __forceinline__ void embed_groups(const Net& n, const float* __restri
     }
 }
 
-// Embeddings of the even lattice: patch (2yy, 2xx) -> index yy * ew2 + xx,
-// ew2 = ceil((W - 7) / 2), for yy < ceil((H - 7) / 2).
+// Embeddings of the lattice: patch (kLat yy, kLat xx) -> index yy * ew2 + xx,
+// ew2 = ceil((W - 7) / kLat), for yy < ceil((H - 7) / kLat).
 __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __restrict__ plane, int pitch, int W, int H,
                                                        _Float16* __restrict__ emb, float* __restrict__ norm) {
-    const int ew2 = (W - 6) >> 1, eh2 = (H - 6) >> 1;
+    const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
     const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
     const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (wave >= per_row * eh2) return;
@@ -468,8 +475,8 @@ __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __res
         n, plane, pitch,
         [&](int g, int r, int& y, int& x) {
             const int xx = xs + 16 * g + r;
-            y = 2 * yy;
-            x = 2 * (xx < ew2 ? xx : ew2 - 1);
+            y = kLat * yy;
+            x = kLat * (xx < ew2 ? xx : ew2 - 1);
             return xx < ew2 ? yy * ew2 + xx : -1;
         },
         emb, norm);
@@ -521,7 +528,9 @@ __global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restri
 constexpr int kPropCols = 16;
 constexpr int kPropRows = 4;
 constexpr int kPropBlocks = kPropRows * kPropCols;
-constexpr int kSubCap = 25 * 25;  // even lattice of the largest window
+constexpr int kSubSide = (49 + kLat - 1) / kLat;  // lattice points per side of the largest window
+// lattice scores of a tile row (the buffer also holds the per-wave lists later)
+constexpr int kSubCap = kSubSide * kSubSide > 625 ? kSubSide * kSubSide : 625;
 constexpr int kListCap = 9 * (kMaxProposals / 2);
 constexpr int kBitWords = (kMaxWindow + 31) / 32;
 
@@ -603,7 +612,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     const int tcol = lin / row_tiles;
     const int bj0 = (lin - tcol * row_tiles) * kPropRows;
     const int bi0 = tcol * kPropCols;
-    const int ew2 = (p.width - 6) >> 1;
+    const int ew2 = lat_count(p.width - 7);
     const int B = 8;
     const int G = p.group;
     const int K = a.topk;
@@ -631,13 +640,13 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             const int bi = bi0 + 4 * q + ii;
             const int ic = grid_pos(bi <= last_bi ? bi : last_bi, p.block_step, p.width - B);
             const int l = search_lo(ic, p.range, 1), rr = search_hi(ic, p.width - B, p.range, 1);
-            lx0[ii] = (l + 1) & ~1;
-            nlx[ii] = bi <= last_bi && lx0[ii] <= rr ? ((rr - lx0[ii]) >> 1) + 1 : 0;
+            lx0[ii] = lat_first(l);
+            nlx[ii] = bi <= last_bi ? lat_span(lx0[ii], rr) : 0;
         }
         const int u_l = search_lo(grid_pos(bi0, p.block_step, p.width - B), p.range, 1);
         const int u_r = search_hi(grid_pos(last_bi, p.block_step, p.width - B), p.width - B, p.range, 1);
-        const int ux0 = (u_l + 1) & ~1;
-        const int nux = ux0 <= u_r ? ((u_r - ux0) >> 1) + 1 : 0;
+        const int ux0 = lat_first(u_l);
+        const int nux = lat_span(ux0, u_r);
         const int ncb = (nux + 15) >> 4;
 #pragma unroll
         for (int m = 0; m < kPropRows; ++m) {
@@ -656,16 +665,16 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             }
             for (int e = threadIdx.x; e < kPropCols * kSubCap; e += 256) (&sub[0][0])[e] = 0xFFFFu;
             __syncthreads();
-            const int ly0 = (tr[m] + 1) & ~1;
-            const int nly = ly0 <= br[m] ? ((br[m] - ly0) >> 1) + 1 : 0;
+            const int ly0 = lat_first(tr[m]);
+            const int nly = lat_span(ly0, br[m]);
             const int ntile = nly * ncb;
             // one step = 16 lattice positions of one lattice row; the next step's
             // embeddings are loaded while this one is scored
             auto load_step = [&](int tl, h4& b0, h4& b1, float& nq) {
                 const int mr = tl / ncb;
                 const int xi = (tl - mr * ncb) * 16 + r;
-                const int x = ux0 + 2 * (xi < nux ? xi : nux - 1);
-                const size_t pq = (size_t)((ly0 >> 1) + mr) * ew2 + (x >> 1);
+                const int x = ux0 + kLat * (xi < nux ? xi : nux - 1);
+                const size_t pq = (size_t)(ly0 / kLat + mr) * ew2 + x / kLat;
                 __builtin_memcpy(&b0, a.emb + pq * kEmb + 4 * q, sizeof(h4));
                 __builtin_memcpy(&b1, a.emb + pq * kEmb + 16 + 4 * q, sizeof(h4));
                 nq = a.norm[pq];
@@ -676,7 +685,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
             for (int tl = wave; tl < ntile; tl += 4) {
                 const int mr = tl / ncb;
                 const int xi = (tl - mr * ncb) * 16 + r;
-                const int x = ux0 + 2 * (xi < nux ? xi : nux - 1);
+                const int x = ux0 + kLat * (xi < nux ? xi : nux - 1);
                 const h4 b0 = n0, b1 = n1;
                 const float nq = nnq;
                 if (tl + 4 < ntile) load_step(tl + 4, n0, n1, nnq);
@@ -686,7 +695,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                 if (xi >= nux) continue;
 #pragma unroll
                 for (int ii = 0; ii < 4; ++ii) {
-                    const int mc = (x - lx0[ii]) >> 1;
+                    const int mc = (x - lx0[ii]) / kLat;
                     if (x < lx0[ii] || mc >= nlx[ii]) continue;
                     float d = __builtin_fmaf(-2.0f, acc[ii], nref[ii] + nq);
                     d = d > 0.0f ? d : 0.0f;
@@ -701,8 +710,8 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                 const int i = grid_pos(bi, p.block_step, p.width - B);
                 const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
                 const int ncols = rr - l + 1;
-                const int lxk = (l + 1) & ~1;
-                const int nlxk = lxk <= rr ? ((rr - lxk) >> 1) + 1 : 0;
+                const int lxk = lat_first(l);
+                const int nlxk = lat_span(lxk, rr);
                 const int nsub = nly * nlxk;
                 const int j = jr[m], t = tr[m], bt = br[m];
                 const int self = (j - t) * ncols + (i - l);
@@ -720,7 +729,7 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                     if (!(sel & (1u << u))) continue;
                     const int e = lane + 64 * u;
                     const int my = e / nlxk;
-                    const int y0 = ly0 + 2 * my, x0 = lxk + 2 * (e - my * nlxk);
+                    const int y0 = ly0 + kLat * my, x0 = lxk + kLat * (e - my * nlxk);
 #pragma unroll
                     for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
@@ -890,7 +899,7 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
 
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st) {
-    const int ew2 = (W - 6) >> 1, eh2 = (H - 6) >> 1;
+    const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
     if (ew2 <= 0 || eh2 <= 0) return hipSuccess;
     const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
     const int waves = per_row * eh2;
