@@ -363,7 +363,7 @@ __device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0,
     c_hi = c_hi > p.nx - 1 ? p.nx - 1 : c_hi;
 }
 
-constexpr int kAggRecCap = 512;    // records buffered per wave between walk and consume
+constexpr int kAggRecCap = 128;    // records buffered per wave between walk and consume (small: occupancy)
 constexpr int kAggRecWords = 3;    // position, member index b*G + z, reference block b
 constexpr int kAggBatch = 8;       // records per load batch of the consume phase
 
