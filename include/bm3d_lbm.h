@@ -3,12 +3,16 @@
  *
  * No counterpart in the reference: its block matching is the exhaustive
  * search of Block::BlockMatchingMulti (include/Block.h:683-729).  This
- * extension proposes candidates with a small patch-embedding network
- * (64 -> 128 ReLU -> 32, BF16 weights, FP32 accumulation on MFMA), re-ranks
- * every proposal by the exact FP32 SSD with the reference's acceptance rule
- * (dist <= thSSE && dist != 0) and key (float(dist * distMul)), and falls back
- * to the exhaustive search for a reference block whose proposals do not
- * yield G-1 acceptable matches.  Filtering and aggregation are unchanged.
+ * extension scores the even-lattice positions (even row and column) of each
+ * search window with a small patch-embedding network (64 -> 128 ReLU -> 32,
+ * BF16 weights, FP32 accumulation on MFMA), proposes the best min(topk, 32)
+ * of them, re-ranks the 3x3 neighbourhoods of the proposals by the exact FP32
+ * SSD with the reference's acceptance rule (dist <= thSSE && dist != 0) and
+ * key (float(dist * distMul)), and falls back to the exhaustive search for a
+ * reference block whose candidates do not yield G-1 acceptable matches.
+ * Windows of at most topk candidates are searched exhaustively.  Filtering
+ * and aggregation are unchanged.  Block size 8, bm_step 1, bm_range <= 24,
+ * group_size <= 64.
  *
  * All plane, pair and match-list pointers are device pointers unless stated.
  * Planes are float32 with a row pitch in elements.  Calls are serialised on
@@ -54,7 +58,7 @@ int bm3d_lbm_forward(bm3d_lbm *m, const float *plane, int pitch, const int *pair
 
 /* Block matching of one plane with the parameters of a Basic/Final filter
  * (block_size 8, bm_step 1): m = NULL runs the exhaustive search, otherwise
- * the top-k proposals (k <= 64) of m are re-ranked.  count/pos/key receive
+ * the proposal path of m with topk (1..64) as above.  count/pos/key receive
  * the match lists in the layout of bm3d_match_lists.  *fallback (host, may be
  * NULL) receives the number of reference blocks that fell back to the
  * exhaustive search; *ms (host, may be NULL) the GPU time of the matching. */

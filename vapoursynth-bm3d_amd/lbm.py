@@ -4,10 +4,13 @@ A patch-embedding network (8x8 patch -> 128 ReLU -> 32, BF16 weights, FP32
 accumulation) is trained on the GPU by the library's own forward, backward
 and Adam kernels so that the squared distance of two embeddings follows the
 SSD of the two patches.  Attached to a Basic/Final filter
-(``clip.filter.set_matcher(model)``) it proposes the top-k candidates of
-every search window, which are re-ranked by the exact FP32 SSD; reference
-blocks whose proposals do not give G-1 acceptable matches fall back to the
-exhaustive search.  No counterpart in the reference plugin.
+(``clip.filter.set_matcher(model, topk)``) it scores the even-lattice positions
+(even row and column) of every search window by embedding distance and
+proposes the best min(topk, 32); their 3x3 neighbourhoods are re-ranked by
+the exact FP32 SSD.  Reference blocks whose candidates do not give G-1
+acceptable matches fall back to the exhaustive search, and windows of at most
+topk candidates are searched exhaustively.  No counterpart in the reference
+plugin.
 
 Training data: the exhaustive match lists of the plain matcher on synthetic
 noisy frames -- every reference block contributes its matches (positives)
