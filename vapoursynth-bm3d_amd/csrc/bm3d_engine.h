@@ -52,7 +52,7 @@ struct DevBuf {
 
 // Work buffers of the learned proposal path of one filter.
 struct LbmWork {
-    DevBuf emb, norm, remb, rnorm, fallback, nfallback;
+    DevBuf emb, norm, remb, rnorm, fallback, nfallback, bits;
 };
 
 }  // namespace bm3d

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
 
 #include "bm3d_params.h"
 
@@ -63,6 +64,7 @@ struct ProposeArgs {
     float* key;
     int* fallback;   // reference blocks left to the exhaustive search
     int* nfallback;  // their number (device counter, cleared by the caller)
+    uint32_t* cand_bits;  // [nx * ny][propose_bits_words / (nx * ny)] candidate bitmaps
 };
 
 hipError_t launch_train_fwd(const Net& n, const TrainBatch& tb, hipStream_t st);
@@ -80,6 +82,7 @@ hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams
                              hipStream_t st);
 constexpr int kMaxLearnedGroupSize = 64;
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
+size_t propose_bits_words(const StageParams& p);
 
 }  // namespace lbm
 }  // namespace bm3d

@@ -529,8 +529,7 @@ constexpr int kPropCols = 16;
 constexpr int kPropRows = 4;
 constexpr int kPropBlocks = kPropRows * kPropCols;
 constexpr int kSubSide = (49 + kLat - 1) / kLat;  // lattice points per side of the largest window
-// lattice scores of a tile row (the buffer also holds the per-wave lists later)
-constexpr int kSubCap = kSubSide * kSubSide > 625 ? kSubSide * kSubSide : 625;
+constexpr int kSubCap = kSubSide * kSubSide;  // lattice scores per block of a tile row
 constexpr int kListCap = 9 * (kMaxProposals / 2);
 constexpr int kBitWords = (kMaxWindow + 31) / 32;
 
@@ -592,8 +591,6 @@ __device__ __forceinline__ uint32_t wave_smallest16(const uint32_t (&v)[N], int 
     return sel;
 }
 
-static_assert(4 * (2 * kListCap + 4 * kMaxLearnedGroupSize) * 4 <= kPropCols * kSubCap * 2,
-              "per-wave lists must fit in the lattice score buffer");
 
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     __shared__ unsigned short sub[kPropCols][kSubCap];  // lattice scores of one tile row
@@ -747,109 +744,126 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
         }
     }
     __syncthreads();
+    // the blocks' candidate bitmaps go to k_rerank
+    for (int e = threadIdx.x; e < kPropBlocks * kBitWords; e += 256) {
+        const int kb = e / kBitWords, w = e - kb * kBitWords;
+        const int m = kb / kPropCols, k = kb - m * kPropCols;
+        if (!rok[m] || bi0 + k > last_bi) continue;
+        a.cand_bits[(size_t)((bj0 + m) * p.nx + bi0 + k) * kBitWords + w] = shortcut ? 0u : bits[kb][w];
+    }
+}
 
-    // ---- per block: candidates, exact re-rank, output
-    uint32_t* scratch = reinterpret_cast<uint32_t*>(&sub[0][0]);  // the lattice scores are dead now
-    uint32_t* cand = scratch + wave * (2 * kListCap + 4 * kMaxLearnedGroupSize);
+// Per reference block (one wave): candidates from the bitmap of k_propose
+// (window scan order), exact re-rank, output.  A separate kernel so that the
+// latency-bound re-rank runs at the occupancy of its own small LDS footprint.
+constexpr int kRerankWaves = 4;
+constexpr int kRerankWords = 2 * kListCap + 4 * kMaxLearnedGroupSize + 64;
+__global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
+    __shared__ uint32_t smem[kRerankWaves][kRerankWords];
+    const StageParams& p = a.p;
+    const int lane = lane_id();
+    const int wave = uniform((int)(threadIdx.x >> 6));
+    const int b = (int)blockIdx.x * kRerankWaves + wave;
+    if (b >= p.nx * p.ny) return;
+    const int B = 8;
+    const int G = p.group;
+    const int K = a.topk;
+    uint32_t* cand = smem[wave];
     uint32_t* keys = cand + kListCap;
     uint32_t* sel_key = keys + kListCap;
     int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * kMaxLearnedGroupSize);
-    for (int kb = wave; kb < kPropBlocks; kb += 4) {
-        const int m = kb / kPropCols, k = kb - m * kPropCols;
-        const int bi = bi0 + k;
-        if (!rok[m] || bi > last_bi) continue;
-        const int j = jr[m], t = tr[m];
-        const int nrows = br[m] - t + 1;
-        const int b = (bj0 + m) * p.nx + bi;
-        const int i = grid_pos(bi, p.block_step, p.width - B);
-        int* cnt_out = a.count + b;
-        int* pos_out = a.pos + (size_t)b * G;
-        float* key_out = a.key + (size_t)b * G;
-        if (shortcut) {
-            if (lane == 0) {
-                *cnt_out = 1;
-                pos_out[0] = (j << 16) | i;
-                key_out[0] = 0.0f;
-            }
-            continue;
-        }
-        const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
-        const int ncols = rr - l + 1, nc = ncols * nrows;
-        const int self = (j - t) * ncols + (i - l);
-        int n = 0;
-        if (nc - 1 <= K) {
-            for (int c0 = 0; c0 < nc; c0 += 64) {
-                const int c = c0 + lane;
-                const bool in = c < nc && c != self;
-                const uint64_t mk = __ballot(in);
-                const int at = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
-                if (in) cand[at] = (uint32_t)c;
-                n += __popcll(mk);
-            }
-        } else {
-            // the set bits of the block's bitmap, in window order
-            const int nw = (nc + 31) >> 5;
-            const uint32_t w0 = lane < nw ? bits[kb][lane] : 0u;
-            const uint32_t w1 = lane + 64 < nw ? bits[kb][lane + 64] : 0u;
-            const int c0n = __popc(w0), c1n = __popc(w1);
-            int incl = c0n, incl1 = c1n;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v0 = __shfl_up(incl, o), v1 = __shfl_up(incl1, o);
-                if (lane >= o) {
-                    incl += v0;
-                    incl1 += v1;
-                }
-            }
-            const int tot0 = __shfl(incl, 63);
-            n = tot0 + __shfl(incl1, 63);
-            int at = incl - c0n;
-            for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
-            at = tot0 + incl1 - c1n;
-            for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
-        }
-        wave_sync_lds();
-        // exact SSD of every candidate, one lane per candidate
-        float rb[64];
-#pragma unroll
-        for (int e = 0; e < 64; ++e) rb[e] = a.plane[(size_t)(j + e / 8) * p.pitch + i + e % 8];
-        int nacc = 0;
-        for (int e0 = 0; e0 < n; e0 += 64) {
-            const int e = e0 + lane;
-            uint32_t kv = 0xFFFFFFFFu;
-            if (e < n) {
-                const int c = (int)cand[e];
-                const int cy = c / ncols, cx = c - cy * ncols;
-                const float dist = ssd8_rows(rb, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
-                if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
-                keys[e] = kv;
-            }
-            nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
-        }
-        wave_sync_lds();
-        if (nacc < G - 1 && nc - 1 > K) {
-            // not enough acceptable candidates: the exhaustive search decides
-            if (lane == 0) {
-                const int at = atomicAdd(a.nfallback, 1);
-                a.fallback[at] = b;
-            }
-            continue;
-        }
-        const int kk = nacc < G - 1 ? nacc : G - 1;
-        // the kk best by (key, list position); list position order = window order
-        wave_select_small<(kListCap + 63) / 64>(keys, n, nacc, kk, sel_key, sel_idx);
-        for (int z = lane; z < kk; z += 64) {
-            const int c = (int)cand[sel_idx[kk + z]];
-            const int cy = c / ncols;
-            pos_out[1 + z] = ((t + cy) << 16) | (l + c - cy * ncols);
-            key_out[1 + z] = __uint_as_float(sel_key[kk + z]);
-        }
+    float* rbs = reinterpret_cast<float*>(sel_idx + 2 * kMaxLearnedGroupSize);
+    const int by = b / p.nx, bi = b - by * p.nx;
+    const int j = grid_pos(by, p.block_step, p.height - B);
+    const int i = grid_pos(bi, p.block_step, p.width - B);
+    int* cnt_out = a.count + b;
+    int* pos_out = a.pos + (size_t)b * G;
+    float* key_out = a.key + (size_t)b * G;
+    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
         if (lane == 0) {
-            *cnt_out = kk + 1;
+            *cnt_out = 1;
             pos_out[0] = (j << 16) | i;
             key_out[0] = 0.0f;
         }
-        wave_sync_lds();  // cand / keys are rewritten for this wave's next block
+        return;
+    }
+    const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
+    const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
+    const int nrows = bt - t + 1;
+    const int ncols = rr - l + 1, nc = ncols * nrows;
+    const int self = (j - t) * ncols + (i - l);
+    rbs[lane] = a.plane[(size_t)(j + (lane >> 3)) * p.pitch + i + (lane & 7)];
+    int n = 0;
+    if (nc - 1 <= K) {
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+            const int c = c0 + lane;
+            const bool in = c < nc && c != self;
+            const uint64_t mk = __ballot(in);
+            const int at = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
+            if (in) cand[at] = (uint32_t)c;
+            n += __popcll(mk);
+        }
+    } else {
+        // the set bits of the block's bitmap, in window order
+        const int nw = (nc + 31) >> 5;
+        const uint32_t* bw = a.cand_bits + (size_t)b * kBitWords;
+        const uint32_t w0 = lane < nw ? bw[lane] : 0u;
+        const uint32_t w1 = lane + 64 < nw ? bw[lane + 64] : 0u;
+        const int c0n = __popc(w0), c1n = __popc(w1);
+        int incl = c0n, incl1 = c1n;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v0 = __shfl_up(incl, o), v1 = __shfl_up(incl1, o);
+            if (lane >= o) {
+                incl += v0;
+                incl1 += v1;
+            }
+        }
+        const int tot0 = __shfl(incl, 63);
+        n = tot0 + __shfl(incl1, 63);
+        int at = incl - c0n;
+        for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
+        at = tot0 + incl1 - c1n;
+        for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
+    }
+    wave_sync_lds();
+    // exact SSD of every candidate, one lane per candidate (reference block
+    // read from LDS by broadcast)
+    int nacc = 0;
+    for (int e0 = 0; e0 < n; e0 += 64) {
+        const int e = e0 + lane;
+        uint32_t kv = 0xFFFFFFFFu;
+        if (e < n) {
+            const int c = (int)cand[e];
+            const int cy = c / ncols, cx = c - cy * ncols;
+            const float dist = ssd8_rows(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
+            if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+            keys[e] = kv;
+        }
+        nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
+    }
+    wave_sync_lds();
+    if (nacc < G - 1 && nc - 1 > K) {
+        // not enough acceptable candidates: the exhaustive search decides
+        if (lane == 0) {
+            const int at = atomicAdd(a.nfallback, 1);
+            a.fallback[at] = b;
+        }
+        return;
+    }
+    const int kk = nacc < G - 1 ? nacc : G - 1;
+    // the kk best by (key, list position); list position order = window order
+    wave_select_small<(kListCap + 63) / 64>(keys, n, nacc, kk, sel_key, sel_idx);
+    for (int z = lane; z < kk; z += 64) {
+        const int c = (int)cand[sel_idx[kk + z]];
+        const int cy = c / ncols;
+        pos_out[1 + z] = ((t + cy) << 16) | (l + c - cy * ncols);
+        key_out[1 + z] = __uint_as_float(sel_key[kk + z]);
+    }
+    if (lane == 0) {
+        *cnt_out = kk + 1;
+        pos_out[0] = (j << 16) | i;
+        key_out[0] = 0.0f;
     }
 }
 
@@ -920,8 +934,12 @@ hipError_t launch_propose(const ProposeArgs& a, hipStream_t st) {
     const int tiles = ((a.p.ny + kPropRows - 1) / kPropRows) * ((a.p.nx + kPropCols - 1) / kPropCols);
     if (tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(k_propose, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, st, a);
+    const int nb = a.p.nx * a.p.ny;
+    hipLaunchKernelGGL(k_rerank, dim3((nb + kRerankWaves - 1) / kRerankWaves), dim3(64 * kRerankWaves), 0, st, a);
     return hipGetLastError();
 }
+
+size_t propose_bits_words(const StageParams& p) { return (size_t)p.nx * p.ny * kBitWords; }
 
 }  // namespace lbm
 }  // namespace bm3d
