@@ -527,7 +527,6 @@ __global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restri
 // searched exhaustively here.
 constexpr int kPropCols = 16;
 constexpr int kPropRows = 4;
-constexpr int kPropBlocks = kPropRows * kPropCols;
 constexpr int kSubSide = (49 + kLat - 1) / kLat;  // lattice points per side of the largest window
 constexpr int kSubCap = kSubSide * kSubSide;  // lattice scores per block of a tile row
 constexpr int kListCap = 9 * (kMaxProposals / 2);
@@ -594,7 +593,7 @@ __device__ __forceinline__ uint32_t wave_smallest16(const uint32_t (&v)[N], int 
 
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     __shared__ unsigned short sub[kPropCols][kSubCap];  // lattice scores of one tile row
-    __shared__ uint32_t bits[kPropBlocks][kBitWords];   // candidates of each block
+    __shared__ uint32_t bits[4][kBitWords];             // candidates of the block each wave marks
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
@@ -628,7 +627,6 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     }
 
     if (!shortcut) {
-        for (int e = threadIdx.x; e < kPropBlocks * kBitWords; e += 256) (&bits[0][0])[e] = 0u;
         // A fragments: lane r <-> block bi0 + r; D rows 4q + ii <-> block bi0 + 4q + ii
         const int bia = bi0 + r <= last_bi ? bi0 + r : last_bi;
         int lx0[4], nlx[4];
@@ -720,7 +718,9 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                     v[u] = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFu;
                 }
                 const uint32_t sel = wave_smallest16<NS>(v, M);
-                uint32_t* bk = bits[m * kPropCols + k];
+                uint32_t* bk = bits[wave];
+                for (int w = lane; w < kBitWords; w += 64) bk[w] = 0u;
+                wave_sync_lds();
 #pragma unroll
                 for (int u = 0; u < NS; ++u) {
                     if (!(sel & (1u << u))) continue;
@@ -736,20 +736,17 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                             const int c = (yy - t) * ncols + (xx - l);
                             if (c == self) continue;
                             __hip_atomic_fetch_or(&bk[c >> 5], 1u << (c & 31), __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
                         }
                 }
+                wave_sync_lds();
+                // the block's candidate bitmap goes to k_rerank
+                uint32_t* out = a.cand_bits + (size_t)(bjr[m] * p.nx + bi) * kBitWords;
+                for (int w = lane; w < kBitWords; w += 64) out[w] = bk[w];
+                wave_sync_lds();  // bk is cleared for this wave's next block
             }
             __syncthreads();  // sub is rewritten by the next tile row
         }
-    }
-    __syncthreads();
-    // the blocks' candidate bitmaps go to k_rerank
-    for (int e = threadIdx.x; e < kPropBlocks * kBitWords; e += 256) {
-        const int kb = e / kBitWords, w = e - kb * kBitWords;
-        const int m = kb / kPropCols, k = kb - m * kPropCols;
-        if (!rok[m] || bi0 + k > last_bi) continue;
-        a.cand_bits[(size_t)((bj0 + m) * p.nx + bi0 + k) * kBitWords + w] = shortcut ? 0u : bits[kb][w];
     }
 }
 
