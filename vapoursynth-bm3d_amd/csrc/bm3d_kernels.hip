@@ -426,16 +426,28 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     float* __restrict__ out_key) {
     extern __shared__ uint32_t smem[];
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
-    int b;
-    if (blist) {
-        b = blockIdx.x * NW + wave;
-        if (b >= uniform(*nlist)) return;
-        b = uniform(blist[b]);
-    } else {
-        b = xcd_item(nb, NW, wave);
-        if (b < 0) return;
-    }
+    (void)blist;
+    (void)nlist;
+    const int b = xcd_item(nb, NW, wave);
+    if (b < 0) return;
     match8_block<KMAX, W>(plane, p, b, smem + (size_t)wave * wave_words, out_count, out_pos, out_key);
+}
+
+// Sparse re-run over the blocks blist[0 .. *nlist): a small grid whose waves
+// stride over the list (its length is known only on the device).
+template <int KMAX, int W, int NW>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) void k_match8_list(
+    const float* __restrict__ plane, StageParams p, int nb, int wave_words, const int* __restrict__ blist,
+    const int* __restrict__ nlist, int* __restrict__ out_count, int* __restrict__ out_pos,
+    float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int wave = uniform(threadIdx.x >> 6);
+    const int n = uniform(*nlist);
+    for (int k = blockIdx.x * NW + wave; k < n && k < nb; k += gridDim.x * NW) {
+        match8_block<KMAX, W>(plane, p, uniform(blist[k]), smem + (size_t)wave * wave_words, out_count, out_pos,
+                              out_key);
+        wave_sync_lds();
+    }
 }
 
 // Frame of global item idx: the last f with starts[f] <= idx (starts[0] = 0,
@@ -1420,13 +1432,18 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
         const size_t wbytes = sh.words * 4;
         const int nw = wbytes * 4 <= 65536 ? 4 : wbytes * 2 <= 65536 ? 2 : wbytes <= 65536 ? 1 : 0;
         if (km != 0 && nw != 0) {
-            // blist: one wave per listed block (at most nb); else xcd_item's grid
-            const int blocks = (((nb + nw - 1) / nw) + 7) / 8 * 8;
+            // xcd_item's grid; a block list: a grid of 4 workgroups per CU striding over it
+            const int full = (((nb + nw - 1) / nw) + 7) / 8 * 8;
+            const int blocks = blist ? (full < device_cus() * 4 ? full : device_cus() * 4) : full;
             const size_t lds = wbytes * nw;
 #define BM3D_M8(K, WD, NW)                                                                                   \
     if (km == K && sh.w == WD && nw == NW) {                                                                 \
-        hipLaunchKernelGGL((k_match8<K, WD, NW>), dim3(blocks), dim3(64 * NW), lds, st, plane, p, nb,        \
-                           (int)sh.words, blist, nlist, count, pos, key);                                    \
+        if (blist)                                                                                           \
+            hipLaunchKernelGGL((k_match8_list<K, WD, NW>), dim3(blocks), dim3(64 * NW), lds, st, plane, p,  \
+                               nb, (int)sh.words, blist, nlist, count, pos, key);                            \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_match8<K, WD, NW>), dim3(blocks), dim3(64 * NW), lds, st, plane, p, nb,    \
+                               (int)sh.words, blist, nlist, count, pos, key);                                \
         return hipGetLastError();                                                                            \
     }
 #define BM3D_M8W(NW) \
