@@ -158,3 +158,24 @@ def test_basic_final_parameter_variants(gpu, kw):
             f = make("Final", gpu.GRAYS, w, h, profile="np", **kw)
             ref = bm3d_frame(f, gpu.GRAYS, noisy[None], clean[None])[0]
         _assert_equal(node.get_frame(0).planes[0], ref, f"{fn} {kw}")
+
+
+def test_concurrent_get_frame_on_one_filter(gpu):
+    """VapourSynth calls GetFrame of an fmParallel filter from several threads
+    for different frames with one shared instance; the library serialises
+    calls per filter.  Outputs must equal a sequential run."""
+    from concurrent.futures import ThreadPoolExecutor
+    frames = [synth.gray_float(48, 64, 10.0, seed=1200 + k)[0] for k in range(8)]
+    clip = gpu.Clip.from_arrays(gpu.GRAYS, [f[None] for f in frames])
+    seq = gpu.Basic(clip, profile="np")
+    want = [np.asarray(seq.get_frame(n).planes[0]) for n in range(8)]
+    par = gpu.Basic(clip, profile="np")
+    fin = gpu.Final(clip, par, profile="np")
+    with ThreadPoolExecutor(4) as pool:
+        got = list(pool.map(lambda n: np.asarray(par.get_frame(n).planes[0]), range(8)))
+        fin_got = list(pool.map(lambda n: np.asarray(fin.get_frame(n).planes[0]), range(8)))
+    for n in range(8):
+        _assert_equal(got[n], want[n], f"threaded Basic frame {n}")
+    fin_seq = gpu.Final(clip, seq, profile="np")
+    for n in range(8):
+        _assert_equal(fin_got[n], np.asarray(fin_seq.get_frame(n).planes[0]), f"threaded Final frame {n}")
