@@ -15,7 +15,7 @@
  *
  * Frames are device planes of float32 with a row pitch in elements.  Calls
  * run on the batch's stream (ordered after the legacy default stream) and
- * return when the outputs are written.
+ * return when the outputs are written; calls on one batch are serialised.
  */
 #ifndef BM3D_BATCH_H
 #define BM3D_BATCH_H

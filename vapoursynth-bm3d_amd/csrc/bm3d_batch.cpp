@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -83,6 +84,7 @@ struct bm3d_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int64_t pixels = 0;
+    std::mutex mu;  // calls on one batch are serialised (one blob, one workspace)
     ~bm3d_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -446,6 +448,7 @@ int bm3d_batch_run(bm3d_batch* b, const float* const* src, float* const* dst, in
     if (!b || !src || !dst) return set_error(BM3D_E_STATE, "bm3d_batch_run: null argument");
     for (int f = 0; f < b->n; ++f)
         if (!src[f] || !dst[f]) return set_error(BM3D_E_ARG, "bm3d_batch_run: null plane");
+    std::lock_guard<std::mutex> lk(b->mu);
     return run_chain(b, src, dst, graph != 0, ms);
 }
 
@@ -461,6 +464,7 @@ int bm3d_batch_set_shape(bm3d_batch* b, const bm3d_batch_shape* shape) {
     if (!b || !shape) return set_error(BM3D_E_STATE, "bm3d_batch_set_shape: null argument");
     BatchShape t{shape->match_waves, shape->match_strip, shape->group_order};
     if (!shape_ok(b, t)) return set_error(BM3D_E_ARG, "bm3d_batch_set_shape: unsupported shape " + shape_name(t));
+    std::lock_guard<std::mutex> lk(b->mu);
     b->shape = t;
     b->drop_graph();
     return BM3D_OK;
@@ -469,6 +473,7 @@ int bm3d_batch_set_shape(bm3d_batch* b, const bm3d_batch_shape* shape) {
 int bm3d_batch_tune(bm3d_batch* b, const float* const* src, float* const* dst, int reps, char* report,
                     size_t report_len) {
     if (!b || !src || !dst) return set_error(BM3D_E_STATE, "bm3d_batch_tune: null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
     reps = std::max(1, reps);
     std::string rep;
     auto time_shape = [&](const BatchShape& t, float& best) -> int {
