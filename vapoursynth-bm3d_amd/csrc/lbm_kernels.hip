@@ -594,6 +594,7 @@ __device__ __forceinline__ uint32_t wave_smallest16(const uint32_t (&v)[N], int 
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     __shared__ unsigned short sub[kPropCols][kSubCap];  // lattice scores of one tile row
     __shared__ uint32_t bits[4][kBitWords];             // candidates of the block each wave marks
+    __shared__ uint32_t comp[4][128];                   // per wave: compacted lattice scores and positions
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
@@ -712,19 +713,17 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                 const int self = (j - t) * ncols + (i - l);
                 constexpr int NS = (kSubCap + 63) / 64;
                 uint32_t v[NS];
+                uint32_t lmin = 0xFFFFu;
 #pragma unroll
                 for (int u = 0; u < NS; ++u) {
                     const int e = lane + 64 * u;
                     v[u] = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFu;
+                    lmin = v[u] < lmin ? v[u] : lmin;
                 }
-                const uint32_t sel = wave_smallest16<NS>(v, M);
                 uint32_t* bk = bits[wave];
                 for (int w = lane; w < kBitWords; w += 64) bk[w] = 0u;
-                wave_sync_lds();
-#pragma unroll
-                for (int u = 0; u < NS; ++u) {
-                    if (!(sel & (1u << u))) continue;
-                    const int e = lane + 64 * u;
+                // marks the 3x3 neighbourhood of lattice position e (self excluded)
+                auto mark = [&](int e) {
                     const int my = e / nlxk;
                     const int y0 = ly0 + kLat * my, x0 = lxk + kLat * (e - my * nlxk);
 #pragma unroll
@@ -738,6 +737,48 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
                             __hip_atomic_fetch_or(&bk[c >> 5], 1u << (c & 31), __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_WAVEFRONT);
                         }
+                };
+                // bound: the M-th smallest lane minimum tb (M lanes hold a value
+                // <= tb, so the M best by (score, position) all score <= tb);
+                // those values are compacted in position order, usually a few
+                // dozen, and the exact selection runs over them alone
+                uint32_t tb = 0;
+                for (int bit = 15; bit >= 0; --bit) {
+                    const uint32_t trial = tb | (1u << bit);
+                    if (__popcll(__ballot(lmin < trial)) < M) tb = trial;
+                }
+                uint32_t* ck = comp[wave];
+                int* ce = reinterpret_cast<int*>(ck + 64);
+                int ncomp = 0;
+#pragma unroll
+                for (int u = 0; u < NS; ++u) {
+                    const bool in = v[u] <= tb && v[u] != 0xFFFFu;
+                    const uint64_t mk = __ballot(in);
+                    const int at = ncomp + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
+                    if (in && at < 64) {
+                        ck[at] = v[u];
+                        ce[at] = lane + 64 * u;
+                    }
+                    ncomp += __popcll(mk);
+                }
+                wave_sync_lds();
+                if (ncomp <= 64) {
+                    const uint32_t cv = lane < ncomp ? ck[lane] : 0xFFFFu;
+                    uint32_t tv = 0;
+                    for (int bit = 15; bit >= 0; --bit) {
+                        const uint32_t trial = tv | (1u << bit);
+                        if (__popcll(__ballot(cv < trial)) < M) tv = trial;
+                    }
+                    const int less = __popcll(__ballot(cv < tv && cv != 0xFFFFu));
+                    const bool eq = cv == tv && cv != 0xFFFFu;
+                    const uint64_t em = __ballot(eq);
+                    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0));
+                    if ((cv < tv && cv != 0xFFFFu) || (eq && rank < M - less)) mark(ce[lane]);
+                } else {
+                    const uint32_t sel = wave_smallest16<NS>(v, M);
+#pragma unroll
+                    for (int u = 0; u < NS; ++u)
+                        if (sel & (1u << u)) mark(lane + 64 * u);
                 }
                 wave_sync_lds();
                 // the block's candidate bitmap goes to k_rerank
