@@ -390,15 +390,21 @@ __device__ __forceinline__ void infer16(const Frags& f, const s4 (&x)[4], float 
     }
 }
 
-// Patch fragments of the lane's patch at (py, px) (packed bf16 conversion).
-__device__ __forceinline__ void patch_frags(const float* __restrict__ plane, int pitch, int py, int px, s4 (&x)[4]) {
+// The lane's part of the patch at (py, px): rows 2 s + q / 2, columns
+// 4 (q % 2) .. +3 for s = 0..3 (q = lane / 16), as loaded.
+__device__ __forceinline__ void patch_load(const float* __restrict__ plane, int pitch, int py, int px,
+                                           float4 (&v)[4]) {
     const int q = lane_id() >> 4;
 #pragma unroll
+    for (int s = 0; s < 4; ++s)
+        __builtin_memcpy(&v[s], plane + (size_t)(py + 2 * s + (q >> 1)) * pitch + px + 4 * (q & 1), sizeof(float4));
+}
+
+// The MFMA fragments of a loaded patch part (packed bf16 conversion).
+__device__ __forceinline__ void patch_cvt(const float4 (&v)[4], s4 (&x)[4]) {
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const float* p = plane + (size_t)(py + 2 * s + (q >> 1)) * pitch + px + 4 * (q & 1);
-        float4 v;
-        __builtin_memcpy(&v, p, sizeof(v));
-        const fl2 a = {v.x, v.y}, b = {v.z, v.w};
+        const fl2 a = {v[s].x, v[s].y}, b = {v[s].z, v[s].w};
         const sh2 lo = __builtin_bit_cast(sh2, __builtin_convertvector(a, bf2));
         const sh2 hi = __builtin_bit_cast(sh2, __builtin_convertvector(b, bf2));
         x[s] = s4{lo[0], lo[1], hi[0], hi[1]};
@@ -432,8 +438,9 @@ __device__ __forceinline__ void store_emb(const float (&e)[2][4], int idx, _Floa
 
 // Each wave embeds kEmbedGroups groups of 16 patches; pos(g, r, y, x)
 // returns the output index of lane r's patch in group g (and its top-left
-// corner), or -1.  The next group's patches are loaded while a group runs.
-constexpr int kEmbedGroups = 8;
+// corner), or -1.  The patches of the next two groups are in flight while a
+// group runs (loaded as floats, converted when their group starts).
+constexpr int kEmbedGroups = 16;
 template <typename Pos>
 __device__ __forceinline__ void embed_groups(const Net& n, const float* __restrict__ plane, int pitch, Pos pos,
                                              _Float16* __restrict__ emb, float* __restrict__ norm) {
@@ -441,23 +448,25 @@ __device__ __forceinline__ void embed_groups(const Net& n, const float* __restri
     Frags f;
     load_frags(n, f);
     int y, x;
-    int idx = pos(0, r, y, x);
-    s4 nf[4];
-    patch_frags(plane, pitch, y, x, nf);
-    for (int g = 0; g < kEmbedGroups; ++g) {
-        if (__ballot(idx >= 0) == 0) break;
-        s4 xf[4];
+    int idq[2];
+    float4 nv[2][4];
+    idq[0] = pos(0, r, y, x);
+    patch_load(plane, pitch, y, x, nv[0]);
+    idq[1] = pos(1, r, y, x);
+    patch_load(plane, pitch, y, x, nv[1]);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) xf[s] = nf[s];
-        const int cur = idx;
-        if (g + 1 < kEmbedGroups) {
-            idx = pos(g + 1, r, y, x);
-            patch_frags(plane, pitch, y, x, nf);
+    for (int g = 0; g < kEmbedGroups; ++g) {
+        if (__ballot(idq[g & 1] >= 0) == 0) break;
+        s4 xf[4];
+        patch_cvt(nv[g & 1], xf);
+        const int cur = idq[g & 1];
+        if (g + 2 < kEmbedGroups) {
+            idq[g & 1] = pos(g + 2, r, y, x);
+            patch_load(plane, pitch, y, x, nv[g & 1]);
         }
         float e[2][4];
         infer16(f, xf, e);
         store_emb(e, cur, emb, norm);
-        if (g + 1 >= kEmbedGroups) break;
     }
 }
 
