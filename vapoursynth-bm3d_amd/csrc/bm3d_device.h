@@ -553,6 +553,86 @@ __device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __
     wave_sync_lds();
 }
 
+// The same records in the same order from per-frame member lists: for each
+// block row ar, ro[ar * (nx + 1) + c] counts the members of frame fr held by
+// the row's blocks before column c (c = nx: the row's total), and
+// rp[ar * nx * G + k] are those members' (packed position, member index
+// b * G + z) in scan order.  A row's members in the reach rectangle are
+// contiguous, so the walk reads only members of frame fr.  B*B <= 64 and at
+// most 64 block rows in the rectangle.
+template <int NP>
+__device__ __forceinline__ void agg_tile_rows(const StageParams& p, const int* __restrict__ ro,
+                                              const int2* __restrict__ rp, int a_lo, int a_hi, int c_lo, int c_hi,
+                                              int y0, int x0, const float* const* c, const float* const* d,
+                                              float* acc, int* rec) {
+    const int lane = lane_id();
+    for (int i = lane; i < NP * 2 * kAggPlane; i += 64) acc[i] = 0.0f;
+    wave_sync_lds();
+    const int B = p.block, G = p.group;
+    const int y1 = y0 + kAggTH - 1, x1 = x0 + kAggTW - 1;
+    const int BB = B * B;
+    const int lc = lane < BB ? lane : 0;
+    const int dy = lc / B, dx = lc - (lc / B) * B;
+    const int rstride = p.nx + 1;
+    int nrec = 0;
+    auto append = [&](int2 pr, bool live) {
+        const int qy = pr.x >> 16, qx = pr.x & 0xffff;
+        const bool ov = live && qy <= y1 && qy + B > y0 && qx <= x1 && qx + B > x0;
+        const uint64_t m = __ballot(ov);
+        const int cnt = __popcll(m);
+        if (cnt == 0) return;
+        if (nrec + cnt > kAggRecCap) {
+            wave_sync_lds();
+            agg_consume_list<NP>(rec, nrec, B, lc, dy, dx, y0, x0, c, d, acc);
+            wave_sync_lds();
+            nrec = 0;
+        }
+        if (ov) {
+            const int at = nrec + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            int* e = rec + at * kAggRecWords;
+            e[0] = pr.x;
+            e[1] = pr.y;
+            e[2] = pr.y / G;
+        }
+        nrec += cnt;
+    };
+    // the rows' members form one list (row-major); lane r < nrows holds row
+    // a_lo + r's start rs, length and exclusive prefix ex over the list; 64
+    // consecutive list entries per step, four steps' loads in flight
+    const int nrows = a_hi - a_lo + 1;  // <= 64 (checked by the caller)
+    int rs = 0, rn = 0;
+    if (lane < nrows) {
+        rs = ro[(a_lo + lane) * rstride + c_lo];
+        rn = ro[(a_lo + lane) * rstride + c_hi + 1] - rs;
+    }
+    int incl = rn;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    const int ex = incl - rn;
+    for (int g0 = 0; g0 < total; g0 += 256) {
+        int2 pr[4];
+        bool live[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int g = g0 + 64 * u + lane;
+            live[u] = g < total;
+            int row = 0;  // the last row whose prefix is <= g
+            for (int r = 1; r < nrows; ++r) row = __builtin_amdgcn_readlane(ex, r) <= g ? r : row;
+            const int idx = __shfl(rs, row) + g - __shfl(ex, row);
+            pr[u] = live[u] ? rp[(size_t)(a_lo + row) * p.nx * G + idx] : int2{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) append(pr[u], live[u]);
+    }
+    wave_sync_lds();
+    agg_consume_list<NP>(rec, nrec, B, lc, dy, dx, y0, x0, c, d, acc);
+    wave_sync_lds();
+}
+
 // Runs agg_tile_lds for the processed planes (k-th processed plane -> its
 // contribution and weight arrays, selected among values so that no
 // kernel-argument array is indexed at run time); returns their number.
@@ -575,6 +655,25 @@ __device__ __forceinline__ int agg_tile_any(const StageParams& p, bool p0, bool 
     else if (np == 2) agg_tile_lds<2>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
     else if (np == 1) agg_tile_lds<1>(p, count, pos, frame, fr, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
     return np;
+}
+
+// agg_tile_rows for the processed planes (selection as in agg_tile_any).
+__device__ __forceinline__ void agg_rows_any(const StageParams& p, bool p0, bool p1, bool p2, const int* ro,
+                                             const int2* rp, int a_lo, int a_hi, int c_lo, int c_hi, int y0, int x0,
+                                             const float* c0, const float* c1, const float* c2, const float* d0,
+                                             const float* d1, const float* d2, float* acc, int* rec) {
+    const int np = (int)p0 + (int)p1 + (int)p2;
+    const float* c[3];
+    const float* d[3];
+    c[0] = p0 ? c0 : (p1 ? c1 : c2);
+    d[0] = p0 ? d0 : (p1 ? d1 : d2);
+    c[1] = (p0 && p1) ? c1 : c2;
+    d[1] = (p0 && p1) ? d1 : d2;
+    c[2] = c2;
+    d[2] = d2;
+    if (np == 3) agg_tile_rows<3>(p, ro, rp, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
+    else if (np == 2) agg_tile_rows<2>(p, ro, rp, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
+    else if (np == 1) agg_tile_rows<1>(p, ro, rp, a_lo, a_hi, c_lo, c_hi, y0, x0, c, d, acc, rec);
 }
 
 }  // namespace bm3d

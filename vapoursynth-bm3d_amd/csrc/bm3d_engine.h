@@ -80,7 +80,7 @@ struct bm3d_filter {
     bm3d::DevBuf in_raw[2][3 * 33];  // [src/ref][plane * frames] staged host input
     bm3d::DevBuf fplane[2][3 * 33];  // converted float planes
     bm3d::DevBuf out_stage[3];
-    bm3d::DevBuf m_count, m_pos, m_key, m_frame, m_order, contrib[3], denw[3], ptrs;
+    bm3d::DevBuf m_count, m_pos, m_key, m_frame, m_order, m_vroff, m_vpair, contrib[3], denw[3], ptrs;
     // learned block-matching proposal (null: exhaustive search)
     bm3d_lbm* lbm = nullptr;
     int lbm_topk = 32;

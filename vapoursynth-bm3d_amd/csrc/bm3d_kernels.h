@@ -112,6 +112,8 @@ struct VStackArgs {
     const float* denw[3];
     float* dst[3];          // base of the (2r+1)*2 slot stack per plane
     size_t dst_pitch[3];    // elements
+    int* roff;              // per-frame member lists (k_vframe_rows), or null: frame filter walk
+    int2* rpair;
 };
 
 struct VAggArgs {
@@ -128,6 +130,8 @@ struct VAggArgs {
 size_t vmatch_lds_bytes(const StageParams& p, int max_cand, int ps_cap, int cat_cap);
 hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st);
 hipError_t launch_vstack(const VStackArgs& a, hipStream_t st);
+size_t vframe_rows_words(const StageParams& p, int nframes);  // roff ints
+size_t vframe_pairs(const StageParams& p, int nframes);       // rpair entries
 hipError_t launch_vagg(const VAggArgs& a, hipStream_t st);
 
 size_t match_wave_words(int max_cand, int B, int G);

@@ -754,6 +754,13 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
             vs.dst_pitch[p] = W;
         }
     }
+    if (B * B <= 64) {
+        // per-frame member lists: each frame slot's walk reads only its members
+        if ((rc = f->m_vroff.ensure(sizeof(int) * vframe_rows_words(sp, T)))) return rc;
+        if ((rc = f->m_vpair.ensure(sizeof(int2) * vframe_pairs(sp, T)))) return rc;
+        vs.roff = f->m_vroff.as<int>();
+        vs.rpair = f->m_vpair.as<int2>();
+    }
     prof_begin(f, kTAggregate, f->wiener ? "k_vstack(final)" : "k_vstack(basic)");
     HIP_TRY(launch_vstack(vs, st));
     prof_end(f, kTAggregate);

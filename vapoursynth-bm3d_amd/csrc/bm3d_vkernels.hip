@@ -425,6 +425,39 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 // frame in exactly that order.  Output slot 2*(radius + f - cur) (num) and
 // +1 (den) of the output stack (VBM3D_Base.h:284-292).
 
+// Per-frame member lists for the stack walk: one wave per (block row ar,
+// frame f).  roff[(f * ny + ar) * (nx + 1) + c] = members of frame f held by
+// the row's blocks before column c (c = nx: the row's total);
+// rpair[(f * nb + ar * nx) * G + k] = (packed position, b * G + z) of those
+// members in scan order (blocks by column, members by z).
+__global__ __launch_bounds__(64) void k_vframe_rows(VStackArgs a) {
+    const int ar = blockIdx.x, f = blockIdx.y;
+    const int lane = lane_id();
+    const int nx = a.p.nx, G = a.p.group;
+    int* ro = a.roff + ((size_t)f * a.p.ny + ar) * (nx + 1);
+    int2* rp = a.rpair + ((size_t)f * nx * a.p.ny + (size_t)ar * nx) * G;
+    int carry = 0;
+    for (int c0 = 0; c0 < nx; c0 += 64) {
+        const int c = c0 + lane;
+        const int b = ar * nx + c;
+        const int n = c < nx ? a.count[b] : 0;
+        int k = 0;
+        for (int z = 0; z < n; ++z) k += a.frame[b * G + z] == f;
+        int incl = k;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        int w = carry + incl - k;
+        if (c < nx) ro[c] = w;
+        for (int z = 0; z < n; ++z)
+            if (a.frame[b * G + z] == f) rp[w++] = int2{a.pos[b * G + z], b * G + z};
+        carry += __shfl(incl, 63);
+    }
+    if (lane == 0) ro[nx] = carry;
+}
+
 __global__ __launch_bounds__(64) void k_vstack(VStackArgs a) {
     __shared__ float acc[kAggLds];
     __shared__ int rec[kAggRecCap * kAggRecWords];
@@ -441,9 +474,15 @@ __global__ __launch_bounds__(64) void k_vstack(VStackArgs a) {
     const int reach = a.p.range + dist * (a.p.ps_range / a.p.ps_step * a.p.ps_step);
     int a_lo, a_hi, c_lo, c_hi;
     reach_rect(a.p, y0, x0, kAggTH, kAggTW, reach, a_lo, a_hi, c_lo, c_hi);
-    agg_tile_any(a.p, a.process[0] != 0, a.process[1] != 0, a.process[2] != 0, a.count, a.pos, a.frame, fr, a_lo,
-                 a_hi, c_lo, c_hi, y0, x0, a.contrib[0], a.contrib[1], a.contrib[2], a.denw[0], a.denw[1],
-                 a.denw[2], acc, rec);
+    if (a.roff && a_hi - a_lo < 64)
+        agg_rows_any(a.p, a.process[0] != 0, a.process[1] != 0, a.process[2] != 0,
+                     a.roff + (size_t)fr * a.p.ny * (a.p.nx + 1), a.rpair + (size_t)fr * a.p.nx * a.p.ny * a.p.group,
+                     a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib[0], a.contrib[1], a.contrib[2], a.denw[0], a.denw[1],
+                     a.denw[2], acc, rec);
+    else
+        agg_tile_any(a.p, a.process[0] != 0, a.process[1] != 0, a.process[2] != 0, a.count, a.pos, a.frame, fr,
+                     a_lo, a_hi, c_lo, c_hi, y0, x0, a.contrib[0], a.contrib[1], a.contrib[2], a.denw[0], a.denw[1],
+                     a.denw[2], acc, rec);
     const int slot = 2 * (a.p.radius + fr - a.cur);
     for (int i = lane; i < kAggTH * kAggTW; i += 64) {
         const int ly = i / kAggTW, lx = i - ly * kAggTW;
@@ -515,7 +554,14 @@ hipError_t launch_vmatch(const VMatchArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+size_t vframe_rows_words(const StageParams& p, int nframes) { return (size_t)nframes * p.ny * (p.nx + 1); }
+size_t vframe_pairs(const StageParams& p, int nframes) { return (size_t)nframes * p.nx * p.ny * p.group; }
+
 hipError_t launch_vstack(const VStackArgs& a, hipStream_t st) {
+    if (a.roff) {
+        if (a.p.block * a.p.block > 64) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_vframe_rows, dim3(a.p.ny, a.nframes), dim3(64), 0, st, a);
+    }
     const int tiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
     hipLaunchKernelGGL(k_vstack, dim3(8 * ((tiles + 7) / 8), a.nframes), dim3(64), 0, st, a);
     return hipGetLastError();
