@@ -56,6 +56,19 @@ def test_vbasic_gray_stacks_and_vaggregate(gpu, radius, profile):
         assert "BM3D_V_radius" not in agg.get_frame(n).props
 
 
+@pytest.mark.parametrize("shape", [
+    dict(block_size=4, block_step=3, group_size=8, bm_range=5),     # B*B <= 64: per-frame member-list walk
+    dict(block_size=11, block_step=6, group_size=8, bm_range=7),    # B*B > 64: the block walk
+])
+def test_vbasic_block_sizes(gpu, shape):
+    T, H, W, radius = 4, 44, 60, 1
+    noisy, _ = synth.moving_video(T, H, W, 15.0, seed=37, planes=1)
+    clip = gpu.Clip.from_arrays(gpu.GRAYS, list(noisy))
+    vb = gpu.VBasic(clip, radius=radius, ps_range=3, **shape)
+    f = make("VBasic", gpu.GRAYS, W, H, radius=radius, ps_range=3, **shape)
+    _check_stacks(vb, f, noisy, noisy, radius, 1)
+
+
 def test_vfinal_yuv_float_and_vaggregate16(gpu):
     T, H, W, radius = 4, 40, 56, 1
     noisy, clean = synth.moving_video(T, H, W, 10.0, seed=32, planes=3)
