@@ -218,8 +218,9 @@ int enqueue_stage(bm3d_batch* b, int which) {
                                     b->order.as<int>(), st));
         HIP_TRY(launch_filter_batch(s.a0, blob_at<BatchFilterFrame>(b, s.off_ff), starts, b->n, b->order.as<int>(),
                                     s.total, b->shape, st));
+        // Gray frames: one processed plane (process[0] only)
         HIP_TRY(launch_aggregate_batch(blob_at<AggArgs>(b, s.off_ag), blob_at<int>(b, b->off_tstart), b->n, b->ttotal,
-                                       b->shape, st));
+                                       1, b->shape, st));
         return BM3D_OK;
     }
     for (int f = 0; f < b->n; ++f) {

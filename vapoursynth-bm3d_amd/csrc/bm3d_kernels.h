@@ -155,7 +155,8 @@ hipError_t launch_filter_batch(const FilterArgs& a0, const BatchFilterFrame* ff,
                                const int* order, int total, const BatchShape& sh, hipStream_t st);
 hipError_t launch_bucket_batch(const int* count, const int* starts, int nframes, int nb, int G, int mode, int* hist,
                                int* order, hipStream_t st);
-hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
+// planes: processed planes of every frame in the batch (LDS is sized for it).
+hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total, int planes,
                                   const BatchShape& sh, hipStream_t st);
 int device_cus();
 int aggregate_tiles(int W, int H);

@@ -1296,9 +1296,20 @@ __device__ __forceinline__ void aggregate_tile(const AggArgs& a, int tile, float
     }
 }
 
+// Processed planes of an aggregation (0..3).
+__host__ __device__ __forceinline__ int agg_planes(const AggArgs& a) {
+    return (a.process[0] != 0) + (a.process[1] != 0) + (a.process[2] != 0);
+}
+
+// PL: the planes the LDS accumulators are sized for (1 for Gray launches:
+// a third of the three-plane LDS, so more waves fit on a CU; 3 otherwise).
+// A frame with more processed planes returns without touching LDS (the host
+// never launches one).
+template <int PL>
 __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
-    __shared__ float acc[kAggLds];
+    __shared__ float acc[PL * 2 * kAggPlane];
     __shared__ int rec[kAggRecCap * kAggRecWords];
+    if (agg_planes(a) > PL) return;
     const int ntiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
@@ -1306,14 +1317,16 @@ __global__ __launch_bounds__(64) void k_aggregate(AggArgs a) {
 }
 
 // Batched form: the output tiles of every frame in one launch.
+template <int PL>
 __global__ __launch_bounds__(64) void k_aggregate_batch(const AggArgs* __restrict__ aa, const int* __restrict__ starts,
                                                         int nframes, int total) {
-    __shared__ float acc[kAggLds];
+    __shared__ float acc[PL * 2 * kAggPlane];
     __shared__ int rec[kAggRecCap * kAggRecWords];
     const int t = xcd_item(total, 1, 0);
     if (t < 0) return;
     const int f = batch_frame(starts, nframes, t);
     const AggArgs a = aa[f];
+    if (agg_planes(a) > PL) return;
     aggregate_tile(a, t - uniform(starts[f]), acc, rec);
 }
 
@@ -1542,7 +1555,10 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
 
 hipError_t launch_aggregate(const AggArgs& a, hipStream_t st) {
     const int tiles = ((a.p.width + kAggTW - 1) / kAggTW) * ((a.p.height + kAggTH - 1) / kAggTH);
-    hipLaunchKernelGGL(k_aggregate, dim3(8 * ((tiles + 7) / 8)), dim3(64), 0, st, a);
+    if (agg_planes(a) <= 1)
+        hipLaunchKernelGGL(k_aggregate<1>, dim3(8 * ((tiles + 7) / 8)), dim3(64), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_aggregate<3>, dim3(8 * ((tiles + 7) / 8)), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1622,11 +1638,15 @@ hipError_t launch_filter_batch(const FilterArgs& a0, const BatchFilterFrame* ff,
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total,
+hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nframes, int total, int planes,
                                   const BatchShape& sh, hipStream_t st) {
     if (total <= 0) return hipSuccess;
+    if (planes < 1 || planes > 3) return hipErrorInvalidValue;
     const int groups = batch_groups(total, 1);
-    hipLaunchKernelGGL(k_aggregate_batch, dim3(groups), dim3(64), 0, st, aa, starts, nframes, total);
+    if (planes == 1)
+        hipLaunchKernelGGL(k_aggregate_batch<1>, dim3(groups), dim3(64), 0, st, aa, starts, nframes, total);
+    else
+        hipLaunchKernelGGL(k_aggregate_batch<3>, dim3(groups), dim3(64), 0, st, aa, starts, nframes, total);
     return hipGetLastError();
 }
 
