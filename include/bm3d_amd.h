@@ -128,6 +128,26 @@ int bm3d_frame_range(const bm3d_filter *f, int n, int *first, int *last);
  * written. */
 int bm3d_get_frame(bm3d_filter *f, int n, const bm3d_frame *src, const bm3d_frame *ref, bm3d_frame *dst);
 
+/* Allocation-free form of bm3d_get_frame for Basic and Final filters.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The call makes no hipMalloc/hipFree or other
+ * allocating call: every scratch buffer (converted input planes, match
+ * lists, group contributions and weights, output staging) comes from one
+ * fixed device area of bm3d_noalloc_scratch_bytes() bytes per GPU, held by
+ * the library until the process ends.  A frame whose scratch exceeds it is
+ * rejected with BM3D_E_UNSUPPORTED before anything runs (e.g. 64 MiB covers
+ * np Final on 256x256 Gray float, or np Basic+Final on 192x192 RGB48).
+ * Results equal bm3d_get_frame's.  All calls on one GPU share its area:
+ * calls that overlap in time on one GPU (on any filters) race, so callers
+ * serialise them.  The learned matcher (bm3d_set_matcher) is not supported
+ * here.  The match lists read by bm3d_debug_matches after such a call live in
+ * the shared area until the next call on that GPU. */
+int bm3d_get_frame_noalloc(bm3d_filter *f, int n, const bm3d_frame *src, const bm3d_frame *ref, bm3d_frame *dst);
+
+/* Size in bytes of the fixed scratch area of bm3d_get_frame_noalloc. */
+size_t bm3d_noalloc_scratch_bytes(void);
+
 /* Releases the filter (the *_Free callbacks, source/VSPlugin.cpp:80-84 etc.). */
 void bm3d_free(bm3d_filter *f);
 

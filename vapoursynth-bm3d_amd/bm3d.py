@@ -72,7 +72,8 @@ class BatchShape(C.Structure):
 
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
            "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_profile",
-           "bm3d_profile_read", "bm3d_debug_matches", "bm3d_debug_vmatches")
+           "bm3d_profile_read", "bm3d_debug_matches", "bm3d_debug_vmatches", "bm3d_get_frame_noalloc",
+           "bm3d_noalloc_scratch_bytes")
 
 _lib = None
 _initialised = False
@@ -100,6 +101,9 @@ def load_library(path: str = LIB_PATH):
     L.bm3d_output_info.argtypes = [C.c_void_p, C.POINTER(ClipInfo)]
     L.bm3d_frame_range.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.bm3d_get_frame.argtypes = [C.c_void_p, C.c_int, C.POINTER(FrameC), C.POINTER(FrameC), C.POINTER(FrameC)]
+    L.bm3d_get_frame_noalloc.argtypes = [C.c_void_p, C.c_int, C.POINTER(FrameC), C.POINTER(FrameC),
+                                         C.POINTER(FrameC)]
+    L.bm3d_noalloc_scratch_bytes.restype = C.c_size_t
     L.bm3d_free.argtypes = [C.c_void_p]
     L.bm3d_free.restype = None
     L.bm3d_last_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
@@ -393,7 +397,9 @@ class _Filter:
         except Exception:
             pass
 
-    def get_frame(self, n: int) -> Frame:
+    def get_frame(self, n: int, noalloc: bool = False) -> Frame:
+        """Output frame n.  noalloc: through bm3d_get_frame_noalloc (Basic and
+        Final only; scratch from the library's fixed device area)."""
         first, last = C.c_int(), C.c_int()
         self.L.bm3d_frame_range(self.h, n, C.byref(first), C.byref(last))
         keep: list = []
@@ -411,7 +417,8 @@ class _Filter:
         planes = _alloc_like(self.out_fmt, self.out_w, self.out_h, device)
         out = Frame(planes, {})
         dst = _frame_to_c(out, keep)
-        rc = self.L.bm3d_get_frame(self.h, n, srcs, refs, C.byref(dst))
+        entry = self.L.bm3d_get_frame_noalloc if noalloc else self.L.bm3d_get_frame
+        rc = entry(self.h, n, srcs, refs, C.byref(dst))
         if rc != 0:
             raise Error(f"bm3d.{self.name}: " + self.L.bm3d_last_error().decode())
         out.props = _props_from_c(dst.props, src0.props)

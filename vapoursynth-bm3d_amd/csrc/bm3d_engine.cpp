@@ -106,7 +106,7 @@ static int probe_rcp(std::vector<float>& table, int& bits) {
 #endif
 }
 
-static int ensure_init() {
+int ensure_init() {
     if (!g_dev.ready) return set_error(BM3D_E_STATE, "bm3d_init has not been called");
     HIP_TRY(hipSetDevice(g_dev.device));
     return BM3D_OK;
@@ -463,6 +463,8 @@ int bm3d_init(int device) {
     HIP_TRY(upload_dctz(cz.data(), 32));
     if (!verify_dct_constants())
         return set_error(BM3D_E_STATE, "bm3d_init: compiled DCT constants differ from this host's cos(); rebuild");
+    int rc = noalloc_setup(device);
+    if (rc) return rc;
     g_dev.ready = true;
     return BM3D_OK;
 }
@@ -573,10 +575,11 @@ void bm3d_free(bm3d_filter* f) { delete f; }
 
 int bm3d_last_stats(const bm3d_filter* f, bm3d_stats* out) {
     if (!f || !out) return set_error(BM3D_E_STATE, "bm3d_last_stats: null handle");
+    std::lock_guard<std::mutex> lk(f->mu);
     bm3d_stats s = f->stats;
-    if (f->last_nb > 0 && f->m_count.p && (f->fn == kBasic || f->fn == kFinal)) {
+    if (f->last_nb > 0 && f->last_count && (f->fn == kBasic || f->fn == kFinal)) {
         std::vector<int> c(f->last_nb);
-        HIP_TRY(hipMemcpy(c.data(), f->m_count.p, sizeof(int) * f->last_nb, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(c.data(), f->last_count, sizeof(int) * f->last_nb, hipMemcpyDeviceToHost));
         s.members = s.members_sq = 0;
         for (int v : c) {
             int n = std::min(v, f->G);
@@ -619,33 +622,40 @@ int bm3d_profile_read(const bm3d_filter* f, int idx, const char** name, double* 
     return BM3D_E_ARG;
 }
 
-int bm3d_debug_matches(const bm3d_filter* f, int64_t b, int* count, int* ys, int* xs, float* keys, int cap) {
-    if (!f) return set_error(BM3D_E_STATE, "bm3d_debug_matches: null handle");
-    if (b < 0 || b >= f->last_nb || !f->m_count.p) return set_error(BM3D_E_ARG, "bm3d_debug_matches: no such block");
+// Match list of block b of the last frame; the caller holds f->mu.
+static int debug_matches_locked(const bm3d_filter* f, int64_t b, int* count, int* ys, int* xs, float* keys, int cap) {
+    if (b < 0 || b >= f->last_nb || !f->last_count) return set_error(BM3D_E_ARG, "bm3d_debug_matches: no such block");
     int n = 0;
-    HIP_TRY(hipMemcpy(&n, f->m_count.as<int>() + b, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&n, f->last_count + b, sizeof(int), hipMemcpyDeviceToHost));
     n = std::min(n, f->G);
     std::vector<int> pos(f->G);
     std::vector<float> k(f->G);
-    HIP_TRY(hipMemcpy(pos.data(), f->m_pos.as<int>() + b * f->G, sizeof(int) * f->G, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(k.data(), f->m_key.as<float>() + b * f->G, sizeof(float) * f->G, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(pos.data(), f->last_pos + b * f->G, sizeof(int) * f->G, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(k.data(), f->last_key + b * f->G, sizeof(float) * f->G, hipMemcpyDeviceToHost));
     *count = n;
     for (int z = 0; z < n && z < cap; ++z) {
-        ys[z] = pos[z] >> 16;
+        ys[z] = (int)((uint32_t)pos[z] >> 16);
         xs[z] = pos[z] & 0xffff;
         keys[z] = k[z];
     }
     return BM3D_OK;
 }
 
+int bm3d_debug_matches(const bm3d_filter* f, int64_t b, int* count, int* ys, int* xs, float* keys, int cap) {
+    if (!f) return set_error(BM3D_E_STATE, "bm3d_debug_matches: null handle");
+    std::lock_guard<std::mutex> lk(f->mu);
+    return debug_matches_locked(f, b, count, ys, xs, keys, cap);
+}
+
 int bm3d_debug_vmatches(const bm3d_filter* f, int64_t b, int* count, int* frames, int* ys, int* xs, float* keys,
                         int cap) {
     if (!f) return set_error(BM3D_E_STATE, "bm3d_debug_vmatches: null handle");
-    if (!f->temporal || !f->m_frame.p) return set_error(BM3D_E_ARG, "bm3d_debug_vmatches: not a VBasic/VFinal filter");
-    int rc = bm3d_debug_matches(f, b, count, ys, xs, keys, cap);
+    std::lock_guard<std::mutex> lk(f->mu);
+    if (!f->temporal || !f->last_frame) return set_error(BM3D_E_ARG, "bm3d_debug_vmatches: not a VBasic/VFinal filter");
+    int rc = debug_matches_locked(f, b, count, ys, xs, keys, cap);
     if (rc) return rc;
     std::vector<int> fr(f->G);
-    HIP_TRY(hipMemcpy(fr.data(), f->m_frame.as<int>() + b * f->G, sizeof(int) * f->G, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(fr.data(), f->last_frame + b * f->G, sizeof(int) * f->G, hipMemcpyDeviceToHost));
     for (int z = 0; z < *count && z < cap; ++z) frames[z] = fr[z];
     return BM3D_OK;
 }

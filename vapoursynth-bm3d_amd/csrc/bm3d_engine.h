@@ -13,6 +13,8 @@
 namespace bm3d {
 
 int set_error(int code, const std::string& msg);
+// Fails unless bm3d_init ran; makes the library's device current.
+int ensure_init();
 void log_warning(const std::string& msg);
 int rcp_bits();
 const float* rcp_dev();
@@ -55,6 +57,28 @@ struct LbmWork {
     DevBuf emb, norm, remb, rnorm, fallback, nfallback, bits;
 };
 
+// Where one frame call takes its device scratch from.  A null Scratch*
+// means the filter's own DevBufs (grown with hipMalloc as needed).
+// Otherwise consecutive 256-byte aligned pieces of a fixed area of cap
+// bytes at base; with dry set nothing is launched, copied or recorded and
+// only `used` advances (the sizing pass of a call that must not allocate).
+struct Scratch {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    bool dry = false;
+};
+
+// `bytes` of device scratch for buffer `buf` of the call (see Scratch).
+int take(Scratch* sc, DevBuf& buf, size_t bytes, void** out);
+template <typename T>
+int take_as(Scratch* sc, DevBuf& buf, size_t bytes, T** out) {
+    void* p = nullptr;
+    int rc = take(sc, buf, bytes, &p);
+    *out = static_cast<T*>(p);
+    return rc;
+}
+
 }  // namespace bm3d
 
 struct bm3d_lbm;
@@ -85,10 +109,17 @@ struct bm3d_filter {
     bm3d_lbm* lbm = nullptr;
     int lbm_topk = 32;
     bm3d::LbmWork lbm_work;
+    const float* ptrs_host[6] = {};  // source of the device plane-pointer table
     hipStream_t stream = nullptr;
-    std::mutex mu;
+    mutable std::mutex mu;
     bm3d_stats stats{};
     int last_nb = 0;
+    // match lists of the last frame (the filter's buffers, or the fixed
+    // scratch area after an allocation-free call)
+    const int* last_count = nullptr;
+    const int* last_pos = nullptr;
+    const float* last_key = nullptr;
+    const int* last_frame = nullptr;
     int last_W = 0, last_H = 0;
     // per-kernel event timing
     static constexpr int kMaxTimers = 8;
@@ -107,6 +138,11 @@ struct bm3d_filter {
 
 namespace bm3d {
 int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst);
+// Basic/Final frame with every scratch buffer taken from sc (non-null).
+int run_bm3d_scratch(bm3d_filter* f, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst, Scratch* sc);
+// Registers the fixed scratch area of the allocation-free entry point for
+// the current device (called by bm3d_init with that device current).
+int noalloc_setup(int device);
 // Timer slots used by the launch sequence.
 enum Timer { kTMatch = 0, kTFilter = 1, kTAggregate = 2, kTConvert = 3, kTVAggregate = 4 };
 void prof_begin(bm3d_filter* f, int t, const char* name);

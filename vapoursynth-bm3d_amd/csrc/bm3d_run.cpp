@@ -15,6 +15,23 @@
 
 namespace bm3d {
 
+int take(Scratch* sc, DevBuf& buf, size_t bytes, void** out) {
+    if (!sc) {
+        int rc = buf.ensure(bytes);
+        if (rc) return rc;
+        *out = buf.p;
+        return BM3D_OK;
+    }
+    const size_t at = (sc->used + 255) & ~(size_t)255;
+    if (!sc->dry && at + bytes > sc->cap)
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: scratch area exhausted (" + std::to_string(at + bytes) +
+                                                 " bytes needed, " + std::to_string(sc->cap) + " available)");
+    // the sizing pass hands out distinct non-null addresses that are never used
+    *out = sc->dry ? reinterpret_cast<void*>((uintptr_t)256 + at) : sc->base + at;
+    sc->used = at + bytes;
+    return BM3D_OK;
+}
+
 // Builds the stage parameters of a Basic/Final filter for a W x H plane.
 StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
     StageParams p{};
@@ -197,19 +214,21 @@ size_t direct_float_pitch(const bm3d_clip_info& vi, const bm3d_frame* const* fra
 }
 
 // Returns a device pointer + element pitch for plane p of frame fr, staging
-// host memory into buf when needed.
+// host memory into scratch (buffer buf of the call) when needed.
 int device_plane(const bm3d_frame& fr, int p, int w, int h, int bps, DevBuf& buf, hipStream_t st,
-                 const void*& ptr, size_t& pitch_elems) {
+                 const void*& ptr, size_t& pitch_elems, Scratch* sc = nullptr) {
     if (fr.mem == 1) {
         ptr = fr.data[p];
         pitch_elems = (size_t)fr.stride[p] / bps;
         return BM3D_OK;
     }
-    int rc = buf.ensure((size_t)w * h * bps);
+    void* d = nullptr;
+    int rc = take(sc, buf, (size_t)w * h * bps, &d);
     if (rc) return rc;
-    HIP_TRY(hipMemcpy2DAsync(buf.p, (size_t)w * bps, fr.data[p], fr.stride[p], (size_t)w * bps, h,
-                             hipMemcpyHostToDevice, st));
-    ptr = buf.p;
+    if (!(sc && sc->dry))
+        HIP_TRY(hipMemcpy2DAsync(d, (size_t)w * bps, fr.data[p], fr.stride[p], (size_t)w * bps, h,
+                                 hipMemcpyHostToDevice, st));
+    ptr = d;
     pitch_elems = w;
     return BM3D_OK;
 }
@@ -304,7 +323,9 @@ int64_t ssd_count(const StageParams& p) {
     return sy * sx - (int64_t)p.ny * p.nx;
 }
 
-int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_frame& d) {
+// sc: where the call's scratch comes from (null: the filter's own buffers).
+int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_frame& d, Scratch* sc) {
+    const bool dry = sc && sc->dry;
     const bm3d_clip_info& vi = f->vi;
     const int W = vi.width, H = vi.height;
     const int bps = bytes_of(vi), bits = bits_code(vi);
@@ -313,7 +334,7 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     const int P = nplanes(vi);
     const bool gray_path = vi.color_family == BM3D_CF_GRAY || (vi.color_family == BM3D_CF_YUV && !f->process[1] && !f->process[2]);
     const bool full = rgb ? true : full_range(s.props);
-    if (s.props.bm3d_opp == 1 && !rgb && f->matrix != 100)
+    if (!dry && s.props.bm3d_opp == 1 && !rgb && f->matrix != 100)
         log_warning(std::string(f->wiener ? "bm3d.Basic/bm3d.Final" : "bm3d.Basic/bm3d.Final") +
                     " - warning: There's a frame property \"BM3D_OPP=1\" indicating opponent color space input. "
                     "You should specify \"matrix=100\" in the filter's argument.");
@@ -325,6 +346,7 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     if (!rgb && !proc[0] && !proc[1] && !proc[2]) {
         // VSProcess::process returns the source frame untouched when no plane
         // is processed (include/Helper.h:697-700)
+        if (dry) return BM3D_OK;
         for (int p = 0; p < P; ++p)
             if ((rc = copy_plane(s, d, p, plane_w(vi, p), plane_h(vi, p), bps, st))) return rc;
         d.props = s.props;
@@ -342,10 +364,9 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
             RgbArgs a{};
             for (int p = 0; p < 3; ++p) {
                 size_t pe;
-                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, a.src[p], pe))) return rc;
+                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, a.src[p], pe, sc))) return rc;
                 a.spitch = pe;
-                if ((rc = f->fplane[which][p].ensure(P_ * H * sizeof(float)))) return rc;
-                a.dst[p] = f->fplane[which][p].p;
+                if ((rc = take(sc, f->fplane[which][p], P_ * H * sizeof(float), &a.dst[p]))) return rc;
             }
             a.dpitch = P_;
             a.src_bits = bits;
@@ -355,8 +376,8 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
             a.clip = 0;
             a.y_only = (which == 1 && !f->wiener) ? 1 : 0;  // RGB2FloatY for a Basic ref
             rgb2opp_coefs(bits, 0, a);
-            HIP_TRY(launch_rgb2opp(a, st));
-            for (int p = 0; p < 3; ++p) (which ? Rf : S)[p] = f->fplane[which][p].as<float>();
+            if (!dry) HIP_TRY(launch_rgb2opp(a, st));
+            for (int p = 0; p < 3; ++p) (which ? Rf : S)[p] = static_cast<const float*>(a.dst[p]);
         }
         if (!f->rdef)
             for (int p = 0; p < 3; ++p) Rf[p] = S[p];
@@ -386,10 +407,11 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
                 if (!need) continue;
                 const void* ptr;
                 size_t pe;
-                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, ptr, pe))) return rc;
-                if ((rc = f->fplane[which][p].ensure(P_ * H * sizeof(float)))) return rc;
-                float* dstp = f->fplane[which][p].as<float>();
-                if (isfloat) {
+                if ((rc = device_plane(fr, p, W, H, bps, f->in_raw[which][p], st, ptr, pe, sc))) return rc;
+                float* dstp = nullptr;
+                if ((rc = take_as(sc, f->fplane[which][p], P_ * H * sizeof(float), &dstp))) return rc;
+                if (dry) {
+                } else if (isfloat) {
                     HIP_TRY(launch_to_float(ptr, 0, pe, dstp, P_, W, H, 1.0f, 0.0f, 0, 0.0f, 1.0f, st));
                 } else {
                     ToFloat t = int2float_coefs(bits, full, p > 0);
@@ -408,22 +430,29 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     const int G = f->G, B = f->B;
     const int side = 2 * (sp.range / sp.step) + 1;
     const int max_cand = side * side;
-    if ((rc = f->m_count.ensure(sizeof(int) * nb))) return rc;
-    if ((rc = f->m_pos.ensure(sizeof(int) * (size_t)nb * G))) return rc;
-    if ((rc = f->m_key.ensure(sizeof(float) * (size_t)nb * G))) return rc;
+    int *m_count = nullptr, *m_pos = nullptr;
+    float* m_key = nullptr;
+    if ((rc = take_as(sc, f->m_count, sizeof(int) * nb, &m_count))) return rc;
+    if ((rc = take_as(sc, f->m_pos, sizeof(int) * (size_t)nb * G, &m_pos))) return rc;
+    if ((rc = take_as(sc, f->m_key, sizeof(float) * (size_t)nb * G, &m_key))) return rc;
     if (match_lds_bytes(max_cand, B, G) > 65536)
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: search window too large for the matching kernel");
-    prof_begin(f, kTMatch, f->wiener ? "k_match(final)" : "k_match(basic)");
-    if (f->lbm) {
-        if ((rc = lbm_match(f->lbm, f->lbm_topk, f->lbm_work, Rf[0], sp, f->m_count.as<int>(), f->m_pos.as<int>(),
-                            f->m_key.as<float>(), st)))
-            return rc;
-    } else {
-        HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, f->m_count.as<int>(), f->m_pos.as<int>(),
-                             f->m_key.as<float>(), st));
+    if (sc && f->lbm)
+        return set_error(BM3D_E_UNSUPPORTED, "bm3d: the learned matcher has no fixed-scratch form");
+    if (!dry) {
+        prof_begin(f, kTMatch, f->wiener ? "k_match(final)" : "k_match(basic)");
+        if (f->lbm) {
+            if ((rc = lbm_match(f->lbm, f->lbm_topk, f->lbm_work, Rf[0], sp, m_count, m_pos, m_key, st))) return rc;
+        } else {
+            HIP_TRY(launch_match(Rf[0], sp, nb, max_cand, m_count, m_pos, m_key, st));
+        }
+        prof_end(f, kTMatch);
+        f->last_nb = nb;
+        f->last_count = m_count;
+        f->last_pos = m_pos;
+        f->last_key = m_key;
+        f->last_frame = nullptr;
     }
-    prof_end(f, kTMatch);
-    f->last_nb = nb;
 
     // --- collaborative filtering
     FilterArgs fa{};
@@ -432,24 +461,24 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     fa.nb = nb;
     fa.wiener = f->wiener;
     for (int p = 0; p < 3; ++p) fa.process[p] = proc[p];
-    fa.count = f->m_count.as<int>();
-    fa.pos = f->m_pos.as<int>();
+    fa.count = m_count;
+    fa.pos = m_pos;
     fa.frame = nullptr;
     // device pointer table: [src p0..p2][ref p0..p2]
-    if ((rc = f->ptrs.ensure(sizeof(float*) * 6))) return rc;
-    const float* hp[6] = {S[0], S[1], S[2], Rf[0], Rf[1], Rf[2]};
-    HIP_TRY(hipMemcpyAsync(f->ptrs.p, hp, sizeof(hp), hipMemcpyHostToDevice, st));
-    const float* const* dp = f->ptrs.as<const float* const>();
+    const float** dp = nullptr;
+    if ((rc = take_as(sc, f->ptrs, sizeof(float*) * 6, &dp))) return rc;
+    // the table's source must outlive the asynchronous copy
+    f->ptrs_host[0] = S[0]; f->ptrs_host[1] = S[1]; f->ptrs_host[2] = S[2];
+    f->ptrs_host[3] = Rf[0]; f->ptrs_host[4] = Rf[1]; f->ptrs_host[5] = Rf[2];
+    if (!dry) HIP_TRY(hipMemcpyAsync(dp, f->ptrs_host, sizeof(float*) * 6, hipMemcpyHostToDevice, st));
     for (int p = 0; p < 3; ++p) {
         fa.src[p] = dp + p;
         fa.ref[p] = dp + 3 + p;
         if (!proc[p]) continue;
         if ((size_t)nb * G >= ((size_t)1 << 31))
             return set_error(BM3D_E_UNSUPPORTED, "bm3d: reference blocks x group_size must stay below 2^31 per frame");
-        if ((rc = f->contrib[p].ensure(sizeof(float) * (size_t)nb * G * B * B))) return rc;
-        if ((rc = f->denw[p].ensure(sizeof(float) * nb))) return rc;
-        fa.contrib[p] = f->contrib[p].as<float>();
-        fa.denw[p] = f->denw[p].as<float>();
+        if ((rc = take_as(sc, f->contrib[p], sizeof(float) * (size_t)nb * G * B * B, &fa.contrib[p]))) return rc;
+        if ((rc = take_as(sc, f->denw[p], sizeof(float) * nb, &fa.denw[p]))) return rc;
     }
     fa.tables = f->tables_dev.as<PlaneTables>();
     fa.rcp_table = rcp_dev();
@@ -459,14 +488,18 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     if (!filter_fits(B, G, f->wiener))
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: block_size/group_size too large for the on-chip group buffer");
     if (f->wiener && rcp_bits() == 0) fa.rcp_shift = 23;
-    prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
-    if (filter_uses_buckets(B, G)) {
-        if ((rc = f->m_order.ensure(sizeof(int) * (size_t)nb))) return rc;
-        HIP_TRY(launch_bucket(fa.count, nb, G, f->m_order.as<int>(), st));
-        fa.order = f->m_order.as<int>();
+    int* order = nullptr;
+    if (filter_uses_buckets(B, G))
+        if ((rc = take_as(sc, f->m_order, sizeof(int) * (size_t)nb, &order))) return rc;
+    if (!dry) {
+        prof_begin(f, kTFilter, f->wiener ? "k_filter(wiener)" : "k_filter(hard)");
+        if (order) {
+            HIP_TRY(launch_bucket(fa.count, nb, G, order, st));
+            fa.order = order;
+        }
+        HIP_TRY(launch_filter(fa, st));
+        prof_end(f, kTFilter);
     }
-    HIP_TRY(launch_filter(fa, st));
-    prof_end(f, kTFilter);
 
     // --- aggregation + divide + output conversion
     AggArgs ag{};
@@ -490,11 +523,11 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
             o.dst[p] = d.data[p];
             o.pitch[p] = (size_t)d.stride[p] / ob;
         } else {
-            if ((rc = f->out_stage[p].ensure((size_t)W * H * ob))) return rc;
-            o.dst[p] = f->out_stage[p].p;
+            if ((rc = take(sc, f->out_stage[p], (size_t)W * H * ob, &o.dst[p]))) return rc;
             o.pitch[p] = W;
         }
     }
+    if (dry) return BM3D_OK;
     o.bits = obits;
     if (rgb) {
         o.mode = kOutRGB;
@@ -686,6 +719,10 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     HIP_TRY(launch_vmatch(va, st));
     prof_end(f, kTMatch);
     f->last_nb = nb;
+    f->last_count = va.count;
+    f->last_pos = va.pos;
+    f->last_key = va.key;
+    f->last_frame = va.frame;
     // --- collaborative filtering over frames
     FilterArgs fa{};
     fa.p = sp;
@@ -863,6 +900,10 @@ int run_vaggregate(bm3d_filter* f, int n, const bm3d_frame* src, bm3d_frame& d) 
 
 }  // namespace
 
+int run_bm3d_scratch(bm3d_filter* f, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst, Scratch* sc) {
+    return run_bm3d(f, *src, ref, *dst, sc);
+}
+
 int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* ref, bm3d_frame* dst) {
     (void)n;
     switch (f->fn) {
@@ -871,7 +912,7 @@ int run_frame(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
         return run_color(f, src[0], *dst);
     case kBasic:
     case kFinal:
-        return run_bm3d(f, src[0], ref, *dst);
+        return run_bm3d(f, src[0], ref, *dst, nullptr);
     case kVBasic:
     case kVFinal:
         return run_vbm3d(f, n, src, ref, *dst);

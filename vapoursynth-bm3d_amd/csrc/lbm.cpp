@@ -309,15 +309,16 @@ int bm3d_matcher_fallbacks(const bm3d_filter* f, int* fallback) {
 
 int bm3d_match_lists(const bm3d_filter* f, int64_t* nb, int* group, int* count, int* pos, float* key) {
     if (!f) return set_error(BM3D_E_ARG, "bm3d_match_lists: null filter");
+    std::lock_guard<std::mutex> lk(f->mu);
     if (nb) *nb = f->last_nb;
     if (group) *group = f->G;
     if (!count && !pos && !key) return BM3D_OK;
-    if (f->last_nb <= 0 || !f->m_count.p) return set_error(BM3D_E_STATE, "bm3d_match_lists: no frame processed yet");
+    if (f->last_nb <= 0 || !f->last_count) return set_error(BM3D_E_STATE, "bm3d_match_lists: no frame processed yet");
     HIP_TRY(hipStreamSynchronize(f->stream));
     const size_t nbz = (size_t)f->last_nb;
-    if (count) HIP_TRY(hipMemcpy(count, f->m_count.p, nbz * sizeof(int), hipMemcpyDeviceToHost));
-    if (pos) HIP_TRY(hipMemcpy(pos, f->m_pos.p, nbz * f->G * sizeof(int), hipMemcpyDeviceToHost));
-    if (key) HIP_TRY(hipMemcpy(key, f->m_key.p, nbz * f->G * sizeof(float), hipMemcpyDeviceToHost));
+    if (count) HIP_TRY(hipMemcpy(count, f->last_count, nbz * sizeof(int), hipMemcpyDeviceToHost));
+    if (pos) HIP_TRY(hipMemcpy(pos, f->last_pos, nbz * f->G * sizeof(int), hipMemcpyDeviceToHost));
+    if (key) HIP_TRY(hipMemcpy(key, f->last_key, nbz * f->G * sizeof(float), hipMemcpyDeviceToHost));
     return BM3D_OK;
 }
 
