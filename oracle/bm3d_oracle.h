@@ -16,7 +16,14 @@
  * REDFT10 = 2N, threshold/Wiener constants vs hand-computed values), hand-
  * derived SSD lane-order cases, brute-force float64 BM3D on tiny frames
  * (scipy DCT, python sort), libstdc++ heap-select semantics on crafted key
- * sequences, and the invariants listed with each function.
+ * sequences, and the invariants listed with each function.  The V-BM3D
+ * functions are pinned in tests/test_oracle_vbm3d_pins.py: or_vblock_match
+ * against a float64 restatement of VBM3D_Base.cpp:540-659 / Block.h:655-827
+ * and against brute force with whole-frame predictive windows;
+ * or_vbm3d_kernel's stack layout (absent-neighbour slots untouched, group
+ * size 1 = the single-frame Basic); or_vaggregate_plane on tagged stacks
+ * with one exact answer; the conversions' rounding offsets at hand-derived
+ * boundary samples.
  */
 #ifndef BM3D_ORACLE_H
 #define BM3D_ORACLE_H
