@@ -930,6 +930,132 @@ int or_bm3d_kernel(const or_filter *f, int nplanes, int W, int H, ptrdiff_t stri
     return 0;
 }
 
+/* or_bm3d_kernel with the matching and collaborative filtering of each band of
+ * reference-block rows spread over nthreads threads; the aggregation stays
+ * serial in the reference's scan order (BM3D_Base.cpp:309-351), so the
+ * result is identical to or_bm3d_kernel's.  Test infrastructure: lets the
+ * full-size parity tests check 1080p frames in seconds. */
+typedef struct or_band_job {
+    const or_filter *f;
+    int nplanes, W, H;
+    ptrdiff_t stride;
+    const float *const *src, *const *ref;
+    const int *gy, *gx;
+    int nx, a0, a1;
+    int next;            /* next row to take (atomic) */
+    int *n;              /* [rows * nx] group sizes */
+    int *ys, *xs;        /* [rows * nx * G] */
+    float *grp;          /* [rows * nx][plane][G * B * B] filtered groups */
+    float *nw, *dw;      /* [rows * nx][plane] */
+} or_band_job;
+
+static void *or_band_worker(void *arg)
+{
+    or_band_job *j = (or_band_job *)arg;
+    const or_filter *f = j->f;
+    const int B = f->p.block_size, G = f->p.group_size;
+    const ptrdiff_t gsz = (ptrdiff_t)G * B * B;
+    float keys[OR_MAX_GROUP];
+    float *gr = (float *)malloc(sizeof(float) * gsz);
+    for (;;) {
+        const int a = __sync_fetch_and_add(&j->next, 1) + j->a0;
+        if (a >= j->a1) break;
+        for (int b = 0; b < j->nx; ++b) {
+            const ptrdiff_t k = (ptrdiff_t)(a - j->a0) * j->nx + b;
+            int *ys = j->ys + k * G, *xs = j->xs + k * G;
+            int n = or_block_match(f, j->ref[0], j->W, j->H, j->stride, j->gy[a], j->gx[b], keys, ys, xs);
+            if (n > G) n = G;
+            j->n[k] = n;
+            for (int p = 0; p < j->nplanes; ++p) {
+                if (!f->p.process[p]) continue;
+                float *gs = j->grp + (k * 3 + p) * gsz;
+                for (int z = 0; z < n; ++z)
+                    or_load_block(j->src[p], j->stride, ys[z], xs[z], B, gs + (ptrdiff_t)z * B * B);
+                if (f->p.wiener) {
+                    for (int z = 0; z < n; ++z)
+                        or_load_block(j->ref[p], j->stride, ys[z], xs[z], B, gr + (ptrdiff_t)z * B * B);
+                    or_final_group(f, p, gs, gr, n, &j->nw[k * 3 + p], &j->dw[k * 3 + p]);
+                } else {
+                    or_basic_group(f, p, gs, n, &j->nw[k * 3 + p], &j->dw[k * 3 + p]);
+                }
+            }
+        }
+    }
+    free(gr);
+    return NULL;
+}
+
+int or_bm3d_kernel_mt(const or_filter *f, int nplanes, int W, int H, ptrdiff_t stride,
+                      float *const *dst, const float *const *src, const float *const *ref, int nthreads)
+{
+    const int B = f->p.block_size, G = f->p.group_size;
+    const ptrdiff_t count = (ptrdiff_t)H * stride, gsz = (ptrdiff_t)G * B * B;
+    if (nthreads < 1) nthreads = 1;
+    float *den[3] = {NULL, NULL, NULL};
+    for (int p = 0; p < nplanes; ++p) {
+        if (!f->p.process[p]) continue;
+        den[p] = (float *)calloc(count, sizeof(float));
+        memset(dst[p], 0, sizeof(float) * count);
+    }
+    int *gy = (int *)malloc(sizeof(int) * (H + 1));
+    int *gx = (int *)malloc(sizeof(int) * (W + 1));
+    int ny = or_ref_grid(H, B, f->p.block_step, gy);
+    int nx = or_ref_grid(W, B, f->p.block_step, gx);
+    const int band = 2 * nthreads;
+    or_band_job j;
+    memset(&j, 0, sizeof(j));
+    j.f = f;
+    j.nplanes = nplanes;
+    j.W = W;
+    j.H = H;
+    j.stride = stride;
+    j.src = src;
+    j.ref = ref;
+    j.gy = gy;
+    j.gx = gx;
+    j.nx = nx;
+    const ptrdiff_t nblk = (ptrdiff_t)band * nx;
+    j.n = (int *)malloc(sizeof(int) * nblk);
+    j.ys = (int *)malloc(sizeof(int) * nblk * G);
+    j.xs = (int *)malloc(sizeof(int) * nblk * G);
+    j.grp = (float *)malloc(sizeof(float) * nblk * 3 * gsz);
+    j.nw = (float *)malloc(sizeof(float) * nblk * 3);
+    j.dw = (float *)malloc(sizeof(float) * nblk * 3);
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int a0 = 0; a0 < ny; a0 += band) {
+        j.a0 = a0;
+        j.a1 = a0 + band < ny ? a0 + band : ny;
+        j.next = 0;
+        for (int t = 0; t < nthreads; ++t) pthread_create(&th[t], NULL, or_band_worker, &j);
+        for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+        for (ptrdiff_t k = 0; k < (ptrdiff_t)(j.a1 - a0) * nx; ++k)
+            for (int p = 0; p < nplanes; ++p) {
+                if (!f->p.process[p]) continue;
+                or_aggregate(dst[p], den[p], stride, j.grp + (k * 3 + p) * gsz, j.n[k], B, j.ys + k * G,
+                             j.xs + k * G, j.nw[k * 3 + p], j.dw[k * 3 + p]);
+            }
+    }
+    for (int p = 0; p < nplanes; ++p) {
+        if (!f->p.process[p]) continue;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                ptrdiff_t o = (ptrdiff_t)y * stride + x;
+                dst[p][o] = dst[p][o] / den[p][o];
+            }
+        free(den[p]);
+    }
+    free(th);
+    free(j.n);
+    free(j.ys);
+    free(j.xs);
+    free(j.grp);
+    free(j.nw);
+    free(j.dw);
+    free(gy);
+    free(gx);
+    return 0;
+}
+
 int or_vbm3d_kernel(const or_filter *f, int nplanes, int nframes, int cur, int W, int H,
                     ptrdiff_t stride, float *const *dst, const float *const *src,
                     const float *const *ref)

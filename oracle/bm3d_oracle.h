@@ -118,6 +118,10 @@ float or_rcp(const or_filter *f, float x);
  * process[p] is set.  All planes W x H with a common stride (elements). */
 int or_bm3d_kernel(const or_filter *f, int nplanes, int W, int H, ptrdiff_t stride,
                    float *const *dst, const float *const *src, const float *const *ref);
+/* The same result with the matching and filtering of each band of block
+ * rows spread over nthreads threads (aggregation stays serial, in order). */
+int or_bm3d_kernel_mt(const or_filter *f, int nplanes, int W, int H, ptrdiff_t stride,
+                      float *const *dst, const float *const *src, const float *const *ref, int nthreads);
 /* VBM3D_Process_Base::Kernel (VBM3D_Base.cpp:410-537).  src/ref:
  * [plane * nframes + frame].  dst: [plane] -> base of the (2r+1)*2 slot stack
  * of this output frame (slot s at dst + s*H*stride).  cur = index of the

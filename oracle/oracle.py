@@ -86,6 +86,8 @@ def lib():
         L.or_rcp.restype = C.c_float
         L.or_bm3d_kernel.argtypes = [C.POINTER(Filter), C.c_int, C.c_int, C.c_int, C.c_ssize_t,
                                      C.POINTER(fp), C.POINTER(fp), C.POINTER(fp)]
+        L.or_bm3d_kernel_mt.argtypes = [C.POINTER(Filter), C.c_int, C.c_int, C.c_int, C.c_ssize_t,
+                                     C.POINTER(fp), C.POINTER(fp), C.POINTER(fp), C.c_int]
         L.or_vbm3d_kernel.argtypes = [C.POINTER(Filter), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_ssize_t, C.POINTER(fp), C.POINTER(fp), C.POINTER(fp)]
         L.or_vaggregate_plane.argtypes = [C.c_int, C.c_int, C.c_ssize_t, C.c_int, C.c_int, C.c_int,
@@ -172,9 +174,11 @@ def make_filter(params, arith=ARITH_SSE2, rcp_mode=RCP_NATIVE, rcp_table=None, r
 
 # --------------------------------------------------------------------------- kernels
 
-def bm3d_kernel(f, src, ref=None):
+def bm3d_kernel(f, src, ref=None, nthreads=1):
     """src/ref: float32 arrays (P, H, W).  Returns float32 (P, H, W); planes
-    that are not processed are returned as zeros."""
+    that are not processed are returned as zeros.  nthreads > 1 spreads the
+    matching and filtering of each band of block rows over threads (same
+    result: the aggregation stays in scan order)."""
     src = np.ascontiguousarray(src, dtype=np.float32)
     ref = src if ref is None else np.ascontiguousarray(ref, dtype=np.float32)
     P, H, W = src.shape
@@ -183,7 +187,10 @@ def bm3d_kernel(f, src, ref=None):
     d = (FP * 3)(*[_fp(dst[k]) for k in range(P)] + [None] * (3 - P))
     s = (FP * 3)(*[_fp(src[k]) for k in range(P)] + [None] * (3 - P))
     r = (FP * 3)(*[_fp(ref[k]) for k in range(P)] + [None] * (3 - P))
-    lib().or_bm3d_kernel(C.byref(f), P, W, H, W, d, s, r)
+    if nthreads > 1:
+        lib().or_bm3d_kernel_mt(C.byref(f), P, W, H, W, d, s, r, int(nthreads))
+    else:
+        lib().or_bm3d_kernel(C.byref(f), P, W, H, W, d, s, r)
     return dst
 
 

@@ -488,3 +488,21 @@ def test_tie_orders_differ_only_at_equal_keys(oracle_lib):
                 for q in np.nonzero((y1 != y2) | (x1 != x2))[0]:
                     assert np.sum(k1 == k1[q]) > 1  # position differs only inside a run of equal keys
     assert differ < 50
+
+
+@pytest.mark.parametrize("fn,profile,planes", [("Basic", "np", 3), ("Final", "np", 3), ("Basic", "fast", 1),
+                                               ("Final", "lc", 1)])
+def test_threaded_kernel_equals_serial(oracle_lib, fn, profile, planes):
+    """or_bm3d_kernel_mt (bands of block rows over threads, aggregation kept
+    in scan order) gives the serial kernel's result bit for bit."""
+    h, w = 88, 120
+    rgb, _ = synth.rgb16(h, w, 10.0, seed=77)
+    src = O.rgb2opp(rgb, 16, 0, False).astype(np.float32)[:planes]
+    fam = "yuv" if planes == 3 else "gray"
+    p = O.resolve(fn, width=w, height=h, color_family=fam, profile=profile, matrix=100 if planes == 3 else None)
+    f = O.make_filter(p, tie_mode=O.TIES_INDEX)
+    ref = np.clip(src * 0.95 + 0.02, -0.5, 1.0).astype(np.float32)
+    a = O.bm3d_kernel(f, src, ref)
+    for nt in (2, 5):
+        b = O.bm3d_kernel(f, src, ref, nthreads=nt)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), nt
