@@ -77,6 +77,8 @@ EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_
 
 _lib = None
 _initialised = False
+_LOGFN = C.CFUNCTYPE(None, C.c_int, C.c_char_p, C.c_void_p)
+_log_cb = None
 
 
 def load_library(path: str = LIB_PATH):
@@ -95,6 +97,8 @@ def load_library(path: str = LIB_PATH):
         pass
     L = C.CDLL(path)
     L.bm3d_init.argtypes = [C.c_int]
+    L.bm3d_set_log.argtypes = [_LOGFN, C.c_void_p]
+    L.bm3d_set_log.restype = None
     L.bm3d_last_error.restype = C.c_char_p
     L.bm3d_create.argtypes = [C.c_char_p, C.POINTER(ClipInfo), C.POINTER(ClipInfo), C.POINTER(Arg), C.c_int,
                               C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
@@ -158,6 +162,20 @@ def init(device: int = 0):
             raise Error("bm3d: " + L.bm3d_last_error().decode())
         _initialised = True
     return L
+
+
+def set_log(fn: Optional[Callable[[int, str], None]]) -> None:
+    """Routes the library's warnings -- vsapi->logMessage(mtWarning, ...) in
+    the reference (include/BM3D_Base.h:202-207, VBM3D_Base.h:265-270,
+    VAggregate.h:153-194) -- to fn(level, message); None restores stderr."""
+    global _log_cb
+    L = load_library()
+    if fn is None:
+        _log_cb = None
+        L.bm3d_set_log(C.cast(None, _LOGFN), None)
+        return
+    _log_cb = _LOGFN(lambda level, msg, user: fn(level, msg.decode()))
+    L.bm3d_set_log(_log_cb, None)
 
 
 # --------------------------------------------------------------------------- clips
