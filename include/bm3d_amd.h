@@ -91,6 +91,18 @@ typedef void (*bm3d_log_fn)(int level, const char *msg, void *user);
  * (VapourSynthPluginInit2, source/VSPlugin.cpp:424-428). */
 int bm3d_init(int device);
 
+/* Arithmetic the filters reproduce by default on this host: 0 = the x86
+ * build's SSE2 paths (lane-order SSD, include/Block.h:582-631; rcpps Wiener
+ * gain, source/BM3D_Final.cpp:80-104), chosen when the host's _mm_rcp_ps can
+ * be tabulated; 1 = the scalar paths of builds without __SSE2__ (row-major
+ * SSD, Block.h:632-643; exact Wiener division, BM3D_Final.cpp:106-112).
+ * Runs the host probe once (no GPU needed); the environment variable
+ * BM3D_AMD_HOST_RCP=none makes the probe report failure.  Basic, Final,
+ * VBasic and VFinal also accept an int argument "arith" (0 or 1) that
+ * overrides the default; it has no counterpart in the reference's argument
+ * list (arith=0 is rejected where the probe failed). */
+int bm3d_host_arith(void);
+
 /* Message of the last error on this thread ("" if none). */
 const char *bm3d_last_error(void);
 

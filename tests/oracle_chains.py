@@ -12,9 +12,9 @@ def family_name(fmt):
     return {1: "gray", 2: "rgb", 3: "yuv"}[fmt.color_family]
 
 
-def make(function, fmt, width, height, tie_mode=O.TIES_INDEX, **kw):
+def make(function, fmt, width, height, tie_mode=O.TIES_INDEX, arith=O.ARITH_SSE2, **kw):
     p = O.resolve(function, width=width, height=height, color_family=family_name(fmt), **kw)
-    return O.make_filter(p, tie_mode=tie_mode)
+    return O.make_filter(p, arith=arith, tie_mode=tie_mode)
 
 
 def bm3d_frame(f, fmt, src, ref=None, full=True):

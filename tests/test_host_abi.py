@@ -8,6 +8,7 @@ VSData::setError does ("bm3d.<Function>: <message>", include/Helper.h:519-523)."
 import ctypes as C
 import os
 import re
+import textwrap
 
 import pytest
 
@@ -85,12 +86,21 @@ CASES = [
     ("RGB2OPP", dict(cf=bm3d.RGB), dict(sample=2),
      "bm3d.RGB2OPP: Invalid 'sample' assigned, must be 0 (integer sample type) or 1 (float sample type)"),
     ("OPP2RGB", dict(cf=bm3d.RGB), {}, "bm3d.OPP2RGB: Invalid input clip, must be of YUV color family"),
+    # no counterpart in the reference: the arithmetic selector and the size
+    # limit of the packed (y << 16 | x) match positions
+    ("Basic", {}, dict(arith=2), 'bm3d.Basic: Invalid "arith" assigned, must be 0 (x86 SSE2 arithmetic) or 1 '
+                                 '(scalar arithmetic)'),
+    ("VFinal", {}, dict(arith=-1), 'bm3d.VFinal: Invalid "arith" assigned, must be 0 (x86 SSE2 arithmetic) or 1 '
+                                   '(scalar arithmetic)'),
+    ("Basic", dict(h=32768), {}, "bm3d.Basic: frames wider than 65535 or taller than 32767 pixels are not supported"),
+    ("Final", dict(w=65536), {}, "bm3d.Final: frames wider than 65535 or taller than 32767 pixels are not supported"),
 ]
 
 
 @pytest.mark.parametrize("name,info_kw,args,msg", CASES)
 def test_argument_errors(name, info_kw, args, msg):
-    rc, err = _create(name, _info(**info_kw), **args)
+    ref = _info(**info_kw) if name in ("Final", "VFinal") else None
+    rc, err = _create(name, _info(**info_kw), ref, **args)
     assert rc == E_ARG
     assert err == msg
 
@@ -116,3 +126,32 @@ def test_valid_arguments_pass_validation(name, kw):
     info = _info(cf=bm3d.RGB) if name == "RGB2OPP" else _info()
     rc, err = _create(name, info, **kw)
     assert rc == E_STATE, err
+
+
+def test_host_arith_default_and_forced_probe_failure():
+    """On this x86 host the rcpps probe tabulates, so the default is the SSE2
+    arithmetic (0).  With BM3D_AMD_HOST_RCP=none the probe reports failure:
+    the default becomes the scalar arithmetic (1), arith=1 is accepted and
+    arith=0 is refused (checked in a fresh process: the probe runs once)."""
+    import subprocess
+    import sys
+    import platform
+    if platform.machine() in ("x86_64", "AMD64"):
+        assert bm3d.host_arith() == 0
+    code = textwrap.dedent("""
+        import sys
+        sys.path.insert(0, %r)
+        sys.path.insert(0, %r)
+        import bm3d, test_host_abi as T
+        print(bm3d.host_arith())
+        print(T._create("Final", T._info(), T._info(), arith=0))
+        print(T._create("Final", T._info(), T._info(), arith=1))
+    """) % (os.path.join(ROOT, "vapoursynth-bm3d_amd"), os.path.join(ROOT, "tests"))
+    env = dict(os.environ, BM3D_AMD_HOST_RCP="none")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().splitlines()
+    assert lines[0] == "1"
+    assert lines[1] == str((E_ARG, 'bm3d.Final: "arith=0" needs the host\'s SSE2 rcpps, which could not be '
+                                   'tabulated on this host'))
+    assert lines[2].startswith(f"({E_STATE},")  # valid: stops at the device step without bm3d_init

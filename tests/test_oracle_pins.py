@@ -506,3 +506,35 @@ def test_threaded_kernel_equals_serial(oracle_lib, fn, profile, planes):
     for nt in (2, 5):
         b = O.bm3d_kernel(f, src, ref, nthreads=nt)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), nt
+
+
+def test_final_scalar_matches_float64_exact_division(oracle_lib):
+    """The scalar build's Final (BM3D_Final.cpp:106-112: w = r^2 / (r^2 + s^2),
+    one sequential L2 sum) on a tiny frame against float64 Wiener filtering
+    with exact division; row-major SSD keys (Block.h:632-643) drive the
+    matching of both."""
+    H, W = 32, 32
+    noisy, clean = synth.gray_float(H, W, 10.0, seed=14)
+    p = O.resolve("Final", width=W, height=H, color_family="gray", profile="fast")
+    f = O.make_filter(p, arith=O.ARITH_SCALAR)
+    got = O.bm3d_kernel(f, noisy[None], clean[None])[0]
+    B = p.block_size
+    num = np.zeros((H, W))
+    den = np.zeros((H, W))
+    for j in _grid(H, B, p.block_step):
+        for i in _grid(W, B, p.block_step):
+            _, ys, xs = O.block_match(f, clean, j, i)
+            n = len(ys)
+            gs = sf.dctn(np.stack([noisy[y:y + B, x:x + B] for y, x in zip(ys, xs)]).astype(np.float64), type=2)
+            gr = sf.dctn(np.stack([clean[y:y + B, x:x + B] for y, x in zip(ys, xs)]).astype(np.float64), type=2)
+            r2 = gr ** 2
+            w = r2 / (r2 + f.wiener_sigma_sqr[0][n - 1])
+            dw = 1.0 / max(float((w ** 2).sum()), 1.1920929e-07)
+            inv = sf.dctn(gs * w, type=3)
+            for z, (y, x) in enumerate(zip(ys, xs)):
+                num[y:y + B, x:x + B] += inv[z] * dw / f.final_amp[n - 1]
+                den[y:y + B, x:x + B] += dw
+    assert np.abs(got - num / den).max() < 2e-5
+    # and it is a different arithmetic from the SSE2 build's
+    sse = O.bm3d_kernel(O.make_filter(p), noisy[None], clean[None])[0]
+    assert (sse.view(np.uint32) != got.view(np.uint32)).any()

@@ -73,7 +73,7 @@ class BatchShape(C.Structure):
 EXPORTS = ("bm3d_init", "bm3d_last_error", "bm3d_set_log", "bm3d_create", "bm3d_output_info",
            "bm3d_frame_range", "bm3d_get_frame", "bm3d_free", "bm3d_last_stats", "bm3d_profile",
            "bm3d_profile_read", "bm3d_debug_matches", "bm3d_debug_vmatches", "bm3d_get_frame_noalloc",
-           "bm3d_noalloc_scratch_bytes")
+           "bm3d_noalloc_scratch_bytes", "bm3d_host_arith")
 
 _lib = None
 _initialised = False
@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
     L = C.CDLL(path)
     L.bm3d_init.argtypes = [C.c_int]
     L.bm3d_set_log.argtypes = [_LOGFN, C.c_void_p]
+    L.bm3d_host_arith.restype = C.c_int
     L.bm3d_set_log.restype = None
     L.bm3d_last_error.restype = C.c_char_p
     L.bm3d_create.argtypes = [C.c_char_p, C.POINTER(ClipInfo), C.POINTER(ClipInfo), C.POINTER(Arg), C.c_int,
@@ -152,6 +153,11 @@ def load_library(path: str = LIB_PATH):
     L.bm3d_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     _lib = L
     return L
+
+
+def host_arith() -> int:
+    """bm3d_host_arith(): 0 = x86 SSE2 arithmetic, 1 = scalar (no GPU needed)."""
+    return int(load_library().bm3d_host_arith())
 
 
 def init(device: int = 0):
@@ -542,40 +548,46 @@ def OPP2RGB(input: Clip, sample: Optional[int] = None) -> Clip:
 def Basic(input: Clip, ref: Optional[Clip] = None, profile: Optional[str] = None, sigma=None,
           block_size: Optional[int] = None, block_step: Optional[int] = None, group_size: Optional[int] = None,
           bm_range: Optional[int] = None, bm_step: Optional[int] = None, th_mse: Optional[float] = None,
-          hard_thr: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+          hard_thr: Optional[float] = None, matrix: Optional[int] = None, arith: Optional[int] = None) -> Clip:
+    """arith (no counterpart in the reference): 0 = the x86 build's SSE2
+    arithmetic, 1 = the scalar arithmetic of builds without SSE2; default:
+    bm3d_host_arith()."""
     return _node("Basic", input, ref, dict(profile=profile, sigma=sigma, block_size=block_size,
                                            block_step=block_step, group_size=group_size, bm_range=bm_range,
-                                           bm_step=bm_step, th_mse=th_mse, hard_thr=hard_thr, matrix=matrix))
+                                           bm_step=bm_step, th_mse=th_mse, hard_thr=hard_thr, matrix=matrix,
+                                           arith=arith))
 
 
 def Final(input: Clip, ref: Clip, profile: Optional[str] = None, sigma=None, block_size: Optional[int] = None,
           block_step: Optional[int] = None, group_size: Optional[int] = None, bm_range: Optional[int] = None,
-          bm_step: Optional[int] = None, th_mse: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+          bm_step: Optional[int] = None, th_mse: Optional[float] = None, matrix: Optional[int] = None,
+          arith: Optional[int] = None) -> Clip:
     return _node("Final", input, ref, dict(profile=profile, sigma=sigma, block_size=block_size,
                                            block_step=block_step, group_size=group_size, bm_range=bm_range,
-                                           bm_step=bm_step, th_mse=th_mse, matrix=matrix))
+                                           bm_step=bm_step, th_mse=th_mse, matrix=matrix, arith=arith))
 
 
 def VBasic(input: Clip, ref: Optional[Clip] = None, profile: Optional[str] = None, sigma=None,
            radius: Optional[int] = None, block_size: Optional[int] = None, block_step: Optional[int] = None,
            group_size: Optional[int] = None, bm_range: Optional[int] = None, bm_step: Optional[int] = None,
            ps_num: Optional[int] = None, ps_range: Optional[int] = None, ps_step: Optional[int] = None,
-           th_mse: Optional[float] = None, hard_thr: Optional[float] = None, matrix: Optional[int] = None) -> Clip:
+           th_mse: Optional[float] = None, hard_thr: Optional[float] = None, matrix: Optional[int] = None,
+           arith: Optional[int] = None) -> Clip:
     return _node("VBasic", input, ref, dict(profile=profile, sigma=sigma, radius=radius, block_size=block_size,
                                             block_step=block_step, group_size=group_size, bm_range=bm_range,
                                             bm_step=bm_step, ps_num=ps_num, ps_range=ps_range, ps_step=ps_step,
-                                            th_mse=th_mse, hard_thr=hard_thr, matrix=matrix))
+                                            th_mse=th_mse, hard_thr=hard_thr, matrix=matrix, arith=arith))
 
 
 def VFinal(input: Clip, ref: Clip, profile: Optional[str] = None, sigma=None, radius: Optional[int] = None,
            block_size: Optional[int] = None, block_step: Optional[int] = None, group_size: Optional[int] = None,
            bm_range: Optional[int] = None, bm_step: Optional[int] = None, ps_num: Optional[int] = None,
            ps_range: Optional[int] = None, ps_step: Optional[int] = None, th_mse: Optional[float] = None,
-           matrix: Optional[int] = None) -> Clip:
+           matrix: Optional[int] = None, arith: Optional[int] = None) -> Clip:
     return _node("VFinal", input, ref, dict(profile=profile, sigma=sigma, radius=radius, block_size=block_size,
                                             block_step=block_step, group_size=group_size, bm_range=bm_range,
                                             bm_step=bm_step, ps_num=ps_num, ps_range=ps_range, ps_step=ps_step,
-                                            th_mse=th_mse, matrix=matrix))
+                                            th_mse=th_mse, matrix=matrix, arith=arith))
 
 
 def VAggregate(input: Clip, radius: Optional[int] = None, sample: Optional[int] = None) -> Clip:

@@ -146,7 +146,7 @@ void fill_stage(bm3d_batch* b, int which, const float* const* src, float* const*
         }
         fa.tables = flt->tables_dev.as<PlaneTables>();
         fa.rcp_table = rcp_dev();
-        fa.rcp_shift = (s.wiener && rcp_bits() == 0) ? 23 : 23 - rcp_bits();
+        fa.rcp_shift = 23 - rcp_bits();
         fa.coef = flt->coef_dev.as<float>();
         fa.coef_off = flt->coef_off_dev.as<int>();
         fa.contrib[0] = b->contrib.as<float>() + (size_t)s0 * s.G * s.B * s.B;
@@ -327,7 +327,7 @@ int bm3d_batch_create(bm3d_batch** out, int nframes, const int* widths, const in
         int64_t total = 0;
         for (int f = 0; f < nframes; ++f) {
             const int W = widths[f], H = heights[f];
-            if (W <= 0 || H <= 0 || pitches[f] < W || W > 65535 || H > 65535)
+            if (W <= 0 || H <= 0 || pitches[f] < W || W > 65535 || H > 32767)
                 return set_error(BM3D_E_ARG, "bm3d_batch_create: frame " + std::to_string(f) +
                                                  ": invalid width, height or pitch");
             if (!(sigmas[f] > 0.0))

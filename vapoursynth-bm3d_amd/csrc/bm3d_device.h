@@ -42,9 +42,20 @@ __device__ __forceinline__ int search_hi(int pos, int last, int range, int step)
 // full chunk are then added one by one in row-major order.  Each operation
 // rounds separately (no FMA).
 template <int B>
-__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch) {
+__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch, bool scalar) {
     constexpr int SW = B - B % 8;
     float dist = 0.0f;
+    if (scalar) {
+        // builds without __SSE2__ (Block.h:632-643): row-major, one accumulator
+#pragma unroll
+        for (int r = 0; r < B; ++r)
+#pragma unroll
+            for (int c = 0; c < B; ++c) {
+                float t = rb[r * B + c] - q[r * pitch + c];
+                dist = dist + t * t;
+            }
+        return dist;
+    }
     if constexpr (SW > 0) {
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
 #pragma unroll
@@ -85,8 +96,8 @@ __device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restr
 }
 
 // Runtime-B variant (reference block held in LDS).
-__device__ inline float ssd_generic(const float* rb, const float* __restrict__ q, int pitch, int B) {
-    const int SW = B - B % 8;
+__device__ inline float ssd_generic(const float* rb, const float* __restrict__ q, int pitch, int B, bool scalar) {
+    const int SW = scalar ? 0 : B - B % 8;  // scalar: every column row-major
     float dist = 0.0f;
     if (SW > 0) {
         float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;

@@ -13,6 +13,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -105,6 +106,30 @@ static int probe_rcp(std::vector<float>& table, int& bits) {
     return -1;
 #endif
 }
+
+// The host probe, run once per process.  BM3D_AMD_HOST_RCP=none makes it
+// report failure, which selects the scalar arithmetic (a host without a
+// tabulable rcpps behaves like the reference's non-SSE2 builds).
+struct HostRcp {
+    int status = -1;   // 0: tabulated
+    int bits = 0;
+    std::vector<float> table;
+};
+static const HostRcp& host_rcp() {
+    static HostRcp h;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char* env = std::getenv("BM3D_AMD_HOST_RCP");
+        if (env && std::strcmp(env, "none") == 0) {
+            h.status = -1;
+            return;
+        }
+        h.status = probe_rcp(h.table, h.bits);
+    });
+    return h;
+}
+
+int host_default_arith() { return host_rcp().status == 0 ? kArithSSE2 : kArithScalar; }
 
 int ensure_init() {
     if (!g_dev.ready) return set_error(BM3D_E_STATE, "bm3d_init has not been called");
@@ -270,6 +295,8 @@ static std::string parse_bm3d(bm3d_filter* f, const Args& args) {
     if ((vi.sample_type == BM3D_ST_INTEGER && vi.bits_per_sample > 16) ||
         (vi.sample_type == BM3D_ST_FLOAT && vi.bits_per_sample != 32))
         return "Invalid input clip, only 8-16 bit integer or 32 bit float formats supported";
+    if (vi.width > 65535 || vi.height > 32767)
+        return "frames wider than 65535 or taller than 32767 pixels are not supported";
     if (f->rdef) {
         const bm3d_clip_info& r = f->rvi;
         if (r.width <= 0 || r.height <= 0 || r.color_family == 0)
@@ -382,6 +409,15 @@ static std::string parse_bm3d(bm3d_filter* f, const Args& args) {
     } else {
         f->lambda = dlambda;
     }
+    // "arith" has no counterpart in the reference's argument list: it selects
+    // which reference build's arithmetic to reproduce (0: x86 SSE2 paths,
+    // 1: the scalar paths of builds without __SSE2__); default: what the
+    // reference would use on this host.
+    if (!args.get_int("arith", f->arith)) f->arith = host_default_arith();
+    else if (f->arith != kArithSSE2 && f->arith != kArithScalar)
+        return "Invalid \"arith\" assigned, must be 0 (x86 SSE2 arithmetic) or 1 (scalar arithmetic)";
+    else if (f->arith == kArithSSE2 && host_default_arith() != kArithSSE2)
+        return "\"arith=0\" needs the host's SSE2 rcpps, which could not be tabulated on this host";
     return "";
 }
 
@@ -433,12 +469,15 @@ int bm3d_init(int device) {
     HIP_TRY(hipSetDevice(device));
     if (g_dev.ready && g_dev.device == device) return BM3D_OK;
     g_dev.device = device;
-    std::vector<float> tbl;
-    int bits = 0;
-    if (probe_rcp(tbl, bits) != 0) {
-        // The host's rcpps is not a function of a mantissa prefix: fall back to
-        // exact division (the reference's non-SSE2 arithmetic) and say so.
-        log_warning("bm3d: host _mm_rcp_ps could not be tabulated; Final uses exact division");
+    const HostRcp& hr = host_rcp();
+    std::vector<float> tbl = hr.table;
+    int bits = hr.bits;
+    if (hr.status != 0) {
+        // No tabulable rcpps on this host: filters default to the reference's
+        // scalar (non-SSE2) arithmetic -- row-major SSD, exact Wiener
+        // division -- which needs no table (a one-entry placeholder is kept so
+        // the device pointer is valid).
+        log_warning("bm3d: host _mm_rcp_ps could not be tabulated; filters use the scalar (non-SSE2) arithmetic");
         bits = 0;
         tbl.assign(1, 1.0f);
     }
@@ -470,6 +509,8 @@ int bm3d_init(int device) {
 }
 
 const char* bm3d_last_error(void) { return g_last_error.c_str(); }
+
+int bm3d_host_arith(void) { return host_default_arith(); }
 
 void bm3d_set_log(bm3d_log_fn fn, void* user) {
     g_log_fn = fn;

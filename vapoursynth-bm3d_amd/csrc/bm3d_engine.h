@@ -18,6 +18,9 @@ int ensure_init();
 void log_warning(const std::string& msg);
 int rcp_bits();
 const float* rcp_dev();
+// Arithmetic the reference build on this host would use: kArithSSE2 when the
+// host's _mm_rcp_ps can be tabulated, else kArithScalar.  Needs no GPU.
+int host_default_arith();
 
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
@@ -95,6 +98,7 @@ struct bm3d_filter {
     double th_mse = 0, lambda = 0;   // th_mse already multiplied by normY
     int radius = 1, ps_num = 2, ps_range = 4, ps_step = 1;
     int matrix = 2;
+    int arith = 0;                   // kArithSSE2 / kArithScalar (non-reference argument "arith")
     int process[3] = {1, 1, 1};
     int sample = 0;                  // RGB2OPP / OPP2RGB / VAggregate
     // constant tables

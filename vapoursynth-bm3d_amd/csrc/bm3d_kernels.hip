@@ -102,8 +102,8 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
             if (!(y == j && x == i)) {
                 const float* q = plane + (size_t)y * p.pitch + x;
                 float dist;
-                if constexpr (BT > 0) dist = ssd_fixed<BT>(rb, q, p.pitch);
-                else dist = ssd_generic(rb, q, p.pitch, B);
+                if constexpr (BT > 0) dist = ssd_fixed<BT>(rb, q, p.pitch, p.arith != 0);
+                else dist = ssd_generic(rb, q, p.pitch, B, p.arith != 0);
                 if (dist <= p.th_sse && dist != 0.0f) {
                     float key = (float)((double)dist * p.dist_mul);
                     kv = __float_as_uint(key);
@@ -222,23 +222,34 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
             float row[W + 7];
 #pragma unroll
             for (int q = 0; q < W + 7; ++q) row[q] = wr[q];
+            if (p.arith) {
+                // builds without __SSE2__ (Block.h:632-643): row-major, one sum
 #pragma unroll
-            for (int k = 0; k < W; ++k) {
+                for (int k = 0; k < W; ++k)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float d = rv[c] - row[k + c];
-                    s[k][c] = s[k][c] + d * d;
-                }
+                    for (int c = 0; c < 8; ++c) {
+                        float d = rv[c] - row[k + c];
+                        s[k][0] = s[k][0] + d * d;
+                    }
+            } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    float d = rv[4 + c] - row[k + 4 + c];
-                    s[k][c] = s[k][c] + d * d;
+                for (int k = 0; k < W; ++k) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        float d = rv[c] - row[k + c];
+                        s[k][c] = s[k][c] + d * d;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        float d = rv[4 + c] - row[k + 4 + c];
+                        s[k][c] = s[k][c] + d * d;
+                    }
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            float dist = 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
+            const float dist = p.arith ? s[k][0] : 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
             const int cx = sx + k;
             const bool valid = active && cx < ncols && !(cy == oy && cx == ox);
             uint32_t kv = 0xFFFFFFFFu;
@@ -607,7 +618,7 @@ __device__ void filter_group(const float* const* src_planes, const float* const*
                              const int* zyx_pos, int n, int B, bool wiener, const PlaneTables& tb,
                              const float* __restrict__ rcp_tbl, int rcp_shift, const float* __restrict__ coef,
                              const int* __restrict__ coff, float* gs, float* gr, float* tmp,
-                             float* __restrict__ contrib, float* __restrict__ denw_out) {
+                             float* __restrict__ contrib, float* __restrict__ denw_out, bool scalar) {
     const int lane = lane_id();
     const int rs = B + 1, ss = B * (B + 1);
     const int BB = B * B;
@@ -640,8 +651,9 @@ __device__ void filter_group(const float* const* src_planes, const float* const*
         // Wiener gain (BM3D_Final.cpp:74-112): flat index e of group[z][y][x];
         // the SSE2 path covers e < size - size%4 with w = r^2 * rcp(r^2 + s^2)
         // and 4 lane sums of w^2 (lane = e % 4); the tail divides exactly.
+        // The scalar build divides exactly everywhere with one sequential sum.
         const int size = n * BB;
-        const int sw = size - size % 4;
+        const int sw = scalar ? 0 : size - size % 4;
         const float sig2 = tb.wiener_sigma_sqr[n - 1];
         for (int e = lane; e < size; e += 64) {
             int z = e / BB, yx = e - z * BB, y = yx / B, x = yx - y * B;
@@ -698,7 +710,7 @@ __global__ __launch_bounds__(64) void k_filter(FilterArgs a) {
     for (int pl = 0; pl < 3; ++pl) {
         if (!a.process[pl]) continue;
         filter_group(a.src[pl], a.ref[pl], a.p.pitch, zyx, n, B, a.wiener, a.tables[pl], a.rcp_table, a.rcp_shift, a.coef,
-                     a.coef_off, gs, gr, tmp, a.contrib[pl] + (size_t)g * G * B * B, a.denw[pl] + g);
+                     a.coef_off, gs, gr, tmp, a.contrib[pl] + (size_t)g * G * B * B, a.denw[pl] + g, a.p.arith != 0);
     }
 }
 
@@ -952,6 +964,22 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
             // w^2 stored chain-major: SSE2 lane c = x % 4 takes the flat
             // indices e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
             float* w2 = gs;
+            float l2 = 0.0f;
+            if (a.p.arith) {
+                // builds without __SSE2__ (BM3D_Final.cpp:106-112): exact
+                // division, w^2 summed in flat order e = z*64 + y*8 + x
+#pragma unroll
+                for (int k = 0; k < N; ++k) {
+                    const float r2 = R[k] * R[k];
+                    const float w = r2 / (r2 + sig2);
+                    X[k] = X[k] * w;
+                    w2[k * 64 + lane] = w * w;
+                }
+                wave_sync_lds();
+                if (lane == 0)
+                    for (int e = 0; e < N * 64; ++e) l2 = l2 + w2[e];
+                l2 = __shfl(l2, 0);
+            } else {
             const int c = x & 3;
             const int ibase = y * 2 + (x >> 2);
 #pragma unroll
@@ -983,7 +1011,8 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
                 }
             }
             const float p0 = __shfl(part, 0), p1 = __shfl(part, 1), p2 = __shfl(part, 2), p3 = __shfl(part, 3);
-            float l2 = 0.0f + (((p0 + p1) + p2) + p3);
+            l2 = 0.0f + (((p0 + p1) + p2) + p3);
+            }
             l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;
             den_w = 1.0f / l2;
             wave_sync_lds();

@@ -13,6 +13,13 @@ constexpr int kMaxGroup = 256;   // group_size range [1, 256] (BM3D_Base.cpp:160
 constexpr int kMaxBlock = 64;    // block_size range [1, 64]  (BM3D_Base.cpp:136)
 constexpr int kRcpBits = 11;     // mantissa bits indexing the Wiener reciprocal table
 
+// Arithmetic flavour of the reference build being reproduced
+// (include/Helper.h:45-59): the x86 build's SSE2 paths (lane-order SSD,
+// Block.h:582-631; rcpps Wiener gain, BM3D_Final.cpp:80-104) or the scalar
+// paths of builds without __SSE2__ (row-major SSD, Block.h:632-643; exact
+// Wiener division and a sequential L2 sum, BM3D_Final.cpp:106-112).
+enum { kArithSSE2 = 0, kArithScalar = 1 };
+
 // Geometry and matching parameters of one filter stage.
 struct StageParams {
     int width, height, pitch;     // plane size and row pitch (floats)
@@ -23,6 +30,7 @@ struct StageParams {
     float th_sse;                 // float(thMSE' * B^2 / 65025)
     double dist_mul;              // 65025 / B^2
     int radius, ps_num, ps_range, ps_step;   // V-BM3D only
+    int arith;                    // kArithSSE2 or kArithScalar
 };
 
 // Per-plane shrinkage constants indexed by actual group size n (1..G).

@@ -66,6 +66,7 @@ StageParams stage_params(const bm3d_filter* f, int W, int H, size_t pitch) {
     p.ps_num = f->ps_num;
     p.ps_range = f->ps_range;
     p.ps_step = f->ps_step;
+    p.arith = f->arith;
     return p;
 }
 
@@ -487,7 +488,6 @@ int run_bm3d(bm3d_filter* f, const bm3d_frame& s, const bm3d_frame* r, bm3d_fram
     fa.coef_off = f->coef_off_dev.as<int>();
     if (!filter_fits(B, G, f->wiener))
         return set_error(BM3D_E_UNSUPPORTED, "bm3d: block_size/group_size too large for the on-chip group buffer");
-    if (f->wiener && rcp_bits() == 0) fa.rcp_shift = 23;
     int* order = nullptr;
     if (filter_uses_buckets(B, G))
         if ((rc = take_as(sc, f->m_order, sizeof(int) * (size_t)nb, &order))) return rc;
@@ -754,7 +754,7 @@ int run_vbm3d(bm3d_filter* f, int n, const bm3d_frame* src, const bm3d_frame* re
     }
     fa.tables = f->tables_dev.as<PlaneTables>();
     fa.rcp_table = rcp_dev();
-    fa.rcp_shift = rcp_bits() == 0 ? 23 : 23 - rcp_bits();
+    fa.rcp_shift = 23 - rcp_bits();
     fa.coef = f->coef_dev.as<float>();
     fa.coef_off = f->coef_off_dev.as<int>();
     if (!filter_fits(B, G, f->wiener))

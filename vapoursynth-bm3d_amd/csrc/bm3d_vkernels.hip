@@ -270,8 +270,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
         wave_sync_lds();
     }
     auto ssd_at = [&](const float* q, int pitch) {
-        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, pitch);
-        else return ssd_generic(rb, q, pitch, B);
+        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, pitch, p.arith != 0);
+        else return ssd_generic(rb, q, pitch, B, p.arith != 0);
     };
 
     int nacc = 0;

@@ -546,8 +546,20 @@ __device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
 }
 
 // SSD of the 8x8 block rb (row-major registers) against the block at q, in
-// the SSE2 lane order of ssd_fixed<8>.
-__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch) {
+// the SSE2 lane order of ssd_fixed<8> (scalar: the row-major order of builds
+// without __SSE2__, Block.h:632-643).
+__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch, bool scalar) {
+    if (scalar) {
+        float dist = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float t = rb[r * 8 + c] - q[r * pitch + c];
+                dist = dist + t * t;
+            }
+        return dist;
+    }
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -883,7 +895,7 @@ __global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
         if (e < n) {
             const int c = (int)cand[e];
             const int cy = c / ncols, cx = c - cy * ncols;
-            const float dist = ssd8_rows(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
+            const float dist = ssd8_rows(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch, p.arith != 0);
             if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
             keys[e] = kv;
         }
