@@ -41,11 +41,11 @@ __device__ __forceinline__ int search_hi(int pos, int last, int range, int step)
 // 8c+4+k; the lanes are folded as ((s0+s1)+s2)+s3; columns beyond the last
 // full chunk are then added one by one in row-major order.  Each operation
 // rounds separately (no FMA).
-template <int B>
-__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch, bool scalar) {
+template <int B, bool SCALAR = false>
+__device__ __forceinline__ float ssd_fixed(const float* rb, const float* __restrict__ q, int pitch) {
     constexpr int SW = B - B % 8;
     float dist = 0.0f;
-    if (scalar) {
+    if constexpr (SCALAR) {
         // builds without __SSE2__ (Block.h:632-643): row-major, one accumulator
 #pragma unroll
         for (int r = 0; r < B; ++r)

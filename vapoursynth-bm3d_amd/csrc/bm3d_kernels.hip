@@ -32,17 +32,15 @@ namespace bm3d {
 
 constexpr int kMatchWaves = 4;
 
+// Reference block b (the body of k_match and k_match_list); smem: the
+// workgroup's LDS, this wave's part at wave * wave_words.
 template <int BT>
-__global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restrict__ plane, StageParams p,
-                                                            int b0, int nb, int max_cand, int wave_words,
-                                                            int* __restrict__ out_count,
-                                                            int* __restrict__ out_pos,
-                                                            float* __restrict__ out_key) {
-    extern __shared__ uint32_t smem[];
+__device__ __forceinline__ void match_block(const float* __restrict__ plane, const StageParams& p, int b,
+                                            int max_cand, int wave_words, uint32_t* smem,
+                                            int* __restrict__ out_count, int* __restrict__ out_pos,
+                                            float* __restrict__ out_key) {
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
-    const int b = b0 + blockIdx.x * kMatchWaves + wave;
-    if (b >= nb) return;
     const int B = BT > 0 ? BT : p.block;
     const int G = p.group;
     // per-wave LDS: candidate keys, selection scratch (2G keys, 2G indices),
@@ -102,7 +100,7 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
             if (!(y == j && x == i)) {
                 const float* q = plane + (size_t)y * p.pitch + x;
                 float dist;
-                if constexpr (BT > 0) dist = ssd_fixed<BT>(rb, q, p.pitch, p.arith != 0);
+                if constexpr (BT > 0) dist = p.arith ? ssd_fixed<BT, true>(rb, q, p.pitch) : ssd_fixed<BT>(rb, q, p.pitch);
                 else dist = ssd_generic(rb, q, p.pitch, B, p.arith != 0);
                 if (dist <= p.th_sse && dist != 0.0f) {
                     float key = (float)((double)dist * p.dist_mul);
@@ -131,6 +129,37 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restr
         int x = l + (c - cy * ncols) * p.step;
         pos_out[1 + e] = (y << 16) | x;
         key_out[1 + e] = __uint_as_float(sel_key[k + e]);
+    }
+}
+
+template <int BT>
+__global__ __launch_bounds__(64 * kMatchWaves) void k_match(const float* __restrict__ plane, StageParams p,
+                                                            int b0, int nb, int max_cand, int wave_words,
+                                                            int* __restrict__ out_count,
+                                                            int* __restrict__ out_pos,
+                                                            float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int b = b0 + blockIdx.x * kMatchWaves + uniform(threadIdx.x >> 6);
+    if (b >= nb) return;
+    match_block<BT>(plane, p, b, max_cand, wave_words, smem, out_count, out_pos, out_key);
+}
+
+// Sparse re-run over the blocks blist[0 .. *nlist) (its length is known only
+// on the device): the waves of a small grid stride over the list.
+template <int BT>
+__global__ __launch_bounds__(64 * kMatchWaves) void k_match_list(const float* __restrict__ plane, StageParams p,
+                                                                 int nb, int max_cand, int wave_words,
+                                                                 const int* __restrict__ blist,
+                                                                 const int* __restrict__ nlist,
+                                                                 int* __restrict__ out_count,
+                                                                 int* __restrict__ out_pos,
+                                                                 float* __restrict__ out_key) {
+    extern __shared__ uint32_t smem[];
+    const int n = uniform(*nlist);
+    for (int k = blockIdx.x * kMatchWaves + uniform(threadIdx.x >> 6); k < n && k < nb;
+         k += gridDim.x * kMatchWaves) {
+        match_block<BT>(plane, p, uniform(blist[k]), max_cand, wave_words, smem, out_count, out_pos, out_key);
+        wave_sync_lds();
     }
 }
 
@@ -222,34 +251,23 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
             float row[W + 7];
 #pragma unroll
             for (int q = 0; q < W + 7; ++q) row[q] = wr[q];
-            if (p.arith) {
-                // builds without __SSE2__ (Block.h:632-643): row-major, one sum
 #pragma unroll
-                for (int k = 0; k < W; ++k)
+            for (int k = 0; k < W; ++k) {
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        float d = rv[c] - row[k + c];
-                        s[k][0] = s[k][0] + d * d;
-                    }
-            } else {
+                for (int c = 0; c < 4; ++c) {
+                    float d = rv[c] - row[k + c];
+                    s[k][c] = s[k][c] + d * d;
+                }
 #pragma unroll
-                for (int k = 0; k < W; ++k) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        float d = rv[c] - row[k + c];
-                        s[k][c] = s[k][c] + d * d;
-                    }
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        float d = rv[4 + c] - row[k + 4 + c];
-                        s[k][c] = s[k][c] + d * d;
-                    }
+                for (int c = 0; c < 4; ++c) {
+                    float d = rv[4 + c] - row[k + 4 + c];
+                    s[k][c] = s[k][c] + d * d;
                 }
             }
         }
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            const float dist = p.arith ? s[k][0] : 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
+            float dist = 0.0f + (((s[k][0] + s[k][1]) + s[k][2]) + s[k][3]);
             const int cx = sx + k;
             const bool valid = active && cx < ncols && !(cy == oy && cx == ox);
             uint32_t kv = 0xFFFFFFFFu;
@@ -917,7 +935,7 @@ __device__ __forceinline__ void gather8(float* g, const float* const* planes, co
 
 // ONE: only plane 0 is processed (Gray, or YUV with the chroma planes
 // passed through), which keeps the other planes' pointers out of registers.
-template <int N, bool WIENER, bool ONE>
+template <int N, bool WIENER, bool ONE, bool SCALAR>
 __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int* zyx, float* gs) {
     const int lane = lane_id();
     const int G = a.p.group;
@@ -965,7 +983,7 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
             // indices e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
             float* w2 = gs;
             float l2 = 0.0f;
-            if (a.p.arith) {
+            if constexpr (SCALAR) {
                 // builds without __SSE2__ (BM3D_Final.cpp:106-112): exact
                 // division, w^2 summed in flat order e = z*64 + y*8 + x
 #pragma unroll
@@ -1032,7 +1050,7 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
 // One wave per group, taken from `order` (groups bucketed by size so that
 // neighbouring waves run the same specialisation).
 // Group g_id (the body of k_filter8 and its batched form).
-template <int GMAX, bool WIENER, bool ONE>
+template <int GMAX, bool WIENER, bool ONE, bool SCALAR>
 __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, float* fsm) {
     const int lane = lane_id();
     const int G = a.p.group;
@@ -1049,7 +1067,7 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
     switch (n) {
 #define BM3D_G8(NN)                                                         \
     case NN:                                                                \
-        if constexpr (NN <= GMAX) group8<NN, WIENER, ONE>(a, g_id, zyx, gs); \
+        if constexpr (NN <= GMAX) group8<NN, WIENER, ONE, SCALAR>(a, g_id, zyx, gs); \
         break;
         BM3D_G8(1) BM3D_G8(2) BM3D_G8(3) BM3D_G8(4) BM3D_G8(5) BM3D_G8(6) BM3D_G8(7) BM3D_G8(8)
         BM3D_G8(9) BM3D_G8(10) BM3D_G8(11) BM3D_G8(12) BM3D_G8(13) BM3D_G8(14) BM3D_G8(15) BM3D_G8(16)
@@ -1061,12 +1079,12 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
     }
 }
 
-template <int GMAX, bool WIENER, bool ONE>
+template <int GMAX, bool WIENER, bool ONE, bool SCALAR>
 __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
     const int slot = xcd_item(a.nb, 1, 0);
     if (slot < 0) return;
-    filter8_group<GMAX, WIENER, ONE>(a, a.order ? a.order[slot] : slot, fsm);
+    filter8_group<GMAX, WIENER, ONE, SCALAR>(a, a.order ? a.order[slot] : slot, fsm);
 }
 
 // Batched form: the groups of every frame (numbered frame by frame, taken in
@@ -1074,7 +1092,7 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
 // holds what all frames share, with count, pos, contrib and denw indexed by
 // the batch-wide group number; ff[f] the frame's pitch, plane pointer table
 // (src p0..p2, ref p0..p2) and tables.  Plane 0 only (Gray frames).
-template <int GMAX, bool WIENER>
+template <int GMAX, bool WIENER, bool SCALAR>
 __global__ __launch_bounds__(64) void k_filter8_batch(FilterArgs a0, const BatchFilterFrame* __restrict__ ff,
                                                       const int* __restrict__ starts, int nframes,
                                                       const int* __restrict__ order, int total) {
@@ -1089,7 +1107,7 @@ __global__ __launch_bounds__(64) void k_filter8_batch(FilterArgs a0, const Batch
     a.src[0] = pt;
     a.ref[0] = pt + 3;
     a.tables = ff[f].tables;
-    filter8_group<GMAX, WIENER, true>(a, gid, fsm);
+    filter8_group<GMAX, WIENER, true, SCALAR>(a, gid, fsm);
 }
 
 // Bucketing of groups by size, so that neighbouring waves run the same
@@ -1468,7 +1486,9 @@ Match8Shape match8_shape(int max_cand, int G) {
 hipError_t launch_match(const float* plane, const StageParams& p, int nb, int max_cand, int* count, int* pos,
                         float* key, hipStream_t st, const int* blist, const int* nlist) {
     if (nb <= 0) return hipSuccess;
-    if (p.block == 8 && p.step == 1) {
+    // the 8x8 register-strip path computes the x86 (SSE2) lane order only;
+    // the scalar flavour runs the generic matcher
+    if (p.block == 8 && p.step == 1 && p.arith == kArithSSE2) {
         const Match8Shape sh = match8_shape(max_cand, p.group);
         const int km = sh.kmax <= 1 ? 1 : sh.kmax <= 2 ? 2 : sh.kmax <= 3 ? 3 : sh.kmax <= 4 ? 4 : 0;
         const size_t wbytes = sh.words * 4;
@@ -1498,10 +1518,19 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
 #undef BM3D_M8
         }
     }
-    if (blist) return hipErrorInvalidValue;  // sparse re-runs need the B = 8 fast path
     int blocks = (nb + kMatchWaves - 1) / kMatchWaves;
     size_t lds = match_lds_bytes(max_cand, p.block, p.group);
     int ww = (int)match_wave_words(max_cand, p.block, p.group);
+    if (blist) {
+        const int grid = blocks < device_cus() * 4 ? blocks : device_cus() * 4;
+        if (p.block == 8)
+            hipLaunchKernelGGL(k_match_list<8>, dim3(grid), dim3(64 * kMatchWaves), lds, st, plane, p, nb, max_cand,
+                               ww, blist, nlist, count, pos, key);
+        else
+            hipLaunchKernelGGL(k_match_list<0>, dim3(grid), dim3(64 * kMatchWaves), lds, st, plane, p, nb, max_cand,
+                               ww, blist, nlist, count, pos, key);
+        return hipGetLastError();
+    }
     if (p.block == 8)
         hipLaunchKernelGGL(k_match<8>, dim3(blocks), dim3(64 * kMatchWaves), lds, st, plane, p, 0, nb, max_cand,
                            ww, count, pos, key);
@@ -1558,23 +1587,31 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
         const size_t lds = filter8_lds_bytes(a.p.group, a.wiener);
         const bool one = !a.process[1] && !a.process[2];
         const int grid = (a.nb + 7) / 8 * 8;  // xcd_item: a multiple of 8
-#define BM3D_F8(GM, W, O)                                                                \
-    if (gm == GM && a.wiener == W && one == O) {                                        \
-        hipLaunchKernelGGL((k_filter8<GM, W, O>), dim3(grid), dim3(64), lds, st, a);     \
+        // the hard threshold is the same in both arithmetic flavours
+        const bool sc = a.wiener && a.p.arith;
+#define BM3D_F8(GM, W, O, S)                                                             \
+    if (gm == GM && a.wiener == W && one == O && sc == S) {                              \
+        hipLaunchKernelGGL((k_filter8<GM, W, O, S>), dim3(grid), dim3(64), lds, st, a);  \
         return hipGetLastError();                                                        \
     }
-        BM3D_F8(8, 0, false)
-        BM3D_F8(16, 0, false)
-        BM3D_F8(32, 0, false)
-        BM3D_F8(8, 1, false)
-        BM3D_F8(16, 1, false)
-        BM3D_F8(32, 1, false)
-        BM3D_F8(8, 0, true)
-        BM3D_F8(16, 0, true)
-        BM3D_F8(32, 0, true)
-        BM3D_F8(8, 1, true)
-        BM3D_F8(16, 1, true)
-        BM3D_F8(32, 1, true)
+        BM3D_F8(8, 0, false, false)
+        BM3D_F8(16, 0, false, false)
+        BM3D_F8(32, 0, false, false)
+        BM3D_F8(8, 1, false, false)
+        BM3D_F8(16, 1, false, false)
+        BM3D_F8(32, 1, false, false)
+        BM3D_F8(8, 0, true, false)
+        BM3D_F8(16, 0, true, false)
+        BM3D_F8(32, 0, true, false)
+        BM3D_F8(8, 1, true, false)
+        BM3D_F8(16, 1, true, false)
+        BM3D_F8(32, 1, true, false)
+        BM3D_F8(8, 1, false, true)
+        BM3D_F8(16, 1, false, true)
+        BM3D_F8(32, 1, false, true)
+        BM3D_F8(8, 1, true, true)
+        BM3D_F8(16, 1, true, true)
+        BM3D_F8(32, 1, true, true)
 #undef BM3D_F8
     }
     size_t lds = filter_lds_bytes(a.p.block, a.p.group);
@@ -1602,7 +1639,7 @@ int device_cus() {
 }
 
 bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape& sh) {
-    if (p.block != 8 || p.step != 1) return false;
+    if (p.block != 8 || p.step != 1 || p.arith != kArithSSE2) return false;
     if (sh.match_strip != 8 && sh.match_strip != 11) return false;
     if (sh.match_waves != 1 && sh.match_waves != 2 && sh.match_waves != 4) return false;
     const Match8Shape ms = match8_shape_w(max_cand, p.group, sh.match_strip);
@@ -1651,18 +1688,22 @@ hipError_t launch_filter_batch(const FilterArgs& a0, const BatchFilterFrame* ff,
     if (a0.p.block != 8 || gm == 0) return hipErrorInvalidValue;
     const size_t lds = filter8_lds_bytes(G, a0.wiener);
     const int groups = batch_groups(total, 1);
-#define BM3D_F8B(GM, WI)                                                                                  \
-    if (gm == GM && (a0.wiener != 0) == WI) {                                                              \
-        hipLaunchKernelGGL((k_filter8_batch<GM, WI>), dim3(groups), dim3(64), lds, st, a0, ff, starts, nframes, \
-                           order, total);                                                                  \
-        return hipGetLastError();                                                                          \
+    const bool sc = a0.wiener && a0.p.arith;
+#define BM3D_F8B(GM, WI, S)                                                                                   \
+    if (gm == GM && (a0.wiener != 0) == WI && sc == S) {                                                       \
+        hipLaunchKernelGGL((k_filter8_batch<GM, WI, S>), dim3(groups), dim3(64), lds, st, a0, ff, starts, nframes, \
+                           order, total);                                                                      \
+        return hipGetLastError();                                                                              \
     }
-    BM3D_F8B(8, false)
-    BM3D_F8B(16, false)
-    BM3D_F8B(32, false)
-    BM3D_F8B(8, true)
-    BM3D_F8B(16, true)
-    BM3D_F8B(32, true)
+    BM3D_F8B(8, false, false)
+    BM3D_F8B(16, false, false)
+    BM3D_F8B(32, false, false)
+    BM3D_F8B(8, true, false)
+    BM3D_F8B(16, true, false)
+    BM3D_F8B(32, true, false)
+    BM3D_F8B(8, true, true)
+    BM3D_F8B(16, true, true)
+    BM3D_F8B(32, true, true)
 #undef BM3D_F8B
     return hipErrorInvalidValue;
 }

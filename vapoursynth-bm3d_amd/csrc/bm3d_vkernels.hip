@@ -200,8 +200,8 @@ __device__ __forceinline__ void stage_window(float* win, int ws, const float* __
     }
 }
 
-template <int BT, bool STAGE>
-__global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
+template <int BT, bool STAGE, bool SCALAR>
+__device__ __forceinline__ void vmatch_body(const VMatchArgs& a) {
     extern __shared__ uint32_t vsm[];
     const int wave = uniform(threadIdx.x >> 6);  // wave-uniform: scalar registers
     const int lane = lane_id();
@@ -270,8 +270,8 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
         wave_sync_lds();
     }
     auto ssd_at = [&](const float* q, int pitch) {
-        if constexpr (BT > 0) return ssd_fixed<BT>(rb, q, pitch, p.arith != 0);
-        else return ssd_generic(rb, q, pitch, B, p.arith != 0);
+        if constexpr (BT > 0) return ssd_fixed<BT, SCALAR>(rb, q, pitch);
+        else return ssd_generic(rb, q, pitch, B, SCALAR);
     };
 
     int nacc = 0;
@@ -424,6 +424,12 @@ __global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
 // that can reach the tile in that frame and accumulates the members of that
 // frame in exactly that order.  Output slot 2*(radius + f - cur) (num) and
 // +1 (den) of the output stack (VBM3D_Base.h:284-292).
+
+template <int BT, bool STAGE>
+__global__ __launch_bounds__(64 * kVMatchWaves) void k_vmatch(VMatchArgs a) {
+    if (a.p.arith) vmatch_body<BT, STAGE, true>(a);
+    else vmatch_body<BT, STAGE, false>(a);
+}
 
 // Per-frame member lists for the stack walk: one wave per (block row ar,
 // frame f).  roff[(f * ny + ar) * (nx + 1) + c] = members of frame f held by

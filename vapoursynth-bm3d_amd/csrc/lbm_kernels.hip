@@ -548,8 +548,9 @@ __device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
 // SSD of the 8x8 block rb (row-major registers) against the block at q, in
 // the SSE2 lane order of ssd_fixed<8> (scalar: the row-major order of builds
 // without __SSE2__, Block.h:632-643).
-__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch, bool scalar) {
-    if (scalar) {
+template <bool SCALAR>
+__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch) {
+    if constexpr (SCALAR) {
         float dist = 0.0f;
 #pragma unroll
         for (int r = 0; r < 8; ++r)
@@ -817,8 +818,8 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
 // latency-bound re-rank runs at the occupancy of its own small LDS footprint.
 constexpr int kRerankWaves = 4;
 constexpr int kRerankWords = 2 * kListCap + 4 * kMaxLearnedGroupSize + 64;
-__global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
-    __shared__ uint32_t smem[kRerankWaves][kRerankWords];
+template <bool SCALAR>
+__device__ __forceinline__ void rerank_body(const ProposeArgs& a, uint32_t (*smem)[kRerankWords]) {
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
@@ -895,7 +896,7 @@ __global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
         if (e < n) {
             const int c = (int)cand[e];
             const int cy = c / ncols, cx = c - cy * ncols;
-            const float dist = ssd8_rows(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch, p.arith != 0);
+            const float dist = ssd8_rows<SCALAR>(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
             if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
             keys[e] = kv;
         }
@@ -924,6 +925,14 @@ __global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
         pos_out[0] = (j << 16) | i;
         key_out[0] = 0.0f;
     }
+}
+
+// One wave per reference block; the arithmetic flavour is chosen once per
+// launch (the LDS lives here so that both bodies share it).
+__global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
+    __shared__ uint32_t smem[kRerankWaves][kRerankWords];
+    if (a.p.arith) rerank_body<true>(a, smem);
+    else rerank_body<false>(a, smem);
 }
 
 // ---------------------------------------------------------------------------
