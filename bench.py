@@ -31,8 +31,7 @@ sys.path.insert(0, os.path.join(ROOT, "vapoursynth-bm3d_amd"))
 import numpy as np  # noqa: E402
 
 FRAMES, W, H, SIGMA = 8, 1920, 1080, 10.0
-PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (matrix) peak
-PEAK_VALU_NONPACKED = 78.6  # FP32 VALU without packed math (FMA = 2 flops)
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 peak (vector and matrix alike; FMA = 2 flops)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_1080p_rgb48_np.json")
 METRIC = "Mpix/s BM3D Basic+Final 1080p RGB; learned-BM speedup at ≤0.05 dB PSNR loss"
 
@@ -52,6 +51,21 @@ def flops_per_frame(stats, B, wiener, planes=3):
     dct = planes * T * (4 * B ** 3 * stats["members"] + 2 * B * B * stats["members_sq"])
     shrink = planes * c * B * B * stats["members"]
     return ssd, dct + shrink
+
+
+def executed_filter_flops(stats, B, wiener, planes=3):
+    """FLOPs the filter kernel executes per frame of a stage (multiply = 1,
+    add = 1, fused multiply-add = 2): the 8-point x/y passes in their even/odd
+    form (64 flops per forward line, 60 per inverse line, 8n lines per pass),
+    the z chains (n^2 per forward column, about n^2 - n/2 per inverse column),
+    the shrinkage (Wiener 6 per coefficient) and the contribution scaling.
+    Computed from the stage's sum of n and n^2 over its groups."""
+    n1, n2 = stats["members"], stats["members_sq"]
+    fwd = 2 * 8 * 64 * n1 + B * B * n2
+    inv = 2 * 8 * 60 * n1 + B * B * (n2 - n1 / 2)
+    T = 2 if wiener else 1
+    elem = (6 if wiener else 0) * B * B * n1 + B * B * n1
+    return planes * (T * fwd + inv + elem)
 
 
 def measured_traffic(timer):
@@ -250,7 +264,48 @@ def batched_leg(reps=3, tune=True):
             "shape": b.shape, "tune": rep.strip().splitlines()}
 
 
-def cpu_baseline(frames, rows, nthreads):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _ssd_per_pixel(width, height, stage_kw):
+    """SSD evaluations per pixel of a stage on a width x height frame (the
+    search-window rules of Block.h:683-707; float only to compare crops)."""
+    def grid(dim, B, s):
+        out, j = [], 0
+        while j < dim - B + s:
+            out.append(min(j, dim - B))
+            j += s
+        return out
+
+    B, step, R = 8, stage_kw["block_step"], stage_kw["bm_range"]
+
+    def axis(pos, last):
+        return min(pos, R) + min(last - pos, R) + 1
+
+    cy = sum(axis(p, height - B) for p in grid(height, B, step))
+    cx = sum(axis(p, width - B) for p in grid(width, B, step))
+    return (cy * cx - len(grid(height, B, step)) * len(grid(width, B, step))) / (width * height)
+
+
+def cpu_baseline(frames, rows, nthreads, share):
+    """The CPU oracle (a C restatement of the reference CPU path: SSE2 SSD,
+    rcpps Wiener, the project's DCT -- FFTW is not on the box) on the same
+    synthetic frames cropped to W x rows, one frame per thread as VapourSynth
+    runs the reference (fmParallel): once with the 8 frames (min(8, cores)
+    threads) and once with `share` frames on `share` threads (the cores one GPU's
+    share of the host offers).  Crops have shorter search windows near their
+    edges, so the SSD evaluations per pixel of crop and full frame are reported
+    beside the figure."""
+    import ctypes.util
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
@@ -262,9 +317,21 @@ def cpu_baseline(frames, rows, nthreads):
     O.chain_rgb48(fb, ff, sample, nthreads)
     dt = time.perf_counter() - t0
     px = sample.shape[0] * rows * W
+    many = np.ascontiguousarray(np.concatenate([sample] * ((share + len(sample) - 1) // len(sample)))[:share])
+    t1 = time.perf_counter()
+    O.chain_rgb48(fb, ff, many, share)
+    dt_share = time.perf_counter() - t1
+    kb, kf = dict(block_step=4, bm_range=16), dict(block_step=3, bm_range=16)
+    ssd_crop = _ssd_per_pixel(W, rows, kb) + _ssd_per_pixel(W, rows, kf)
+    ssd_full = _ssd_per_pixel(W, H, kb) + _ssd_per_pixel(W, H, kf)
     return {"value": round(px / dt / 1e6, 4), "unit": "Mpix/s", "cores": nthreads, "kind": "port",
             "sample": f"{sample.shape[0]} frames cropped to {W}x{rows} (same synthetic frames), "
-                      f"one frame per thread, {dt:.1f} s wall"}
+                      f"one frame per thread, {dt:.1f} s wall",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "dct_backend": "project DCT (the oracle's DCT-II/III chains; libfftw3f "
+                           + ("present but not used)" if ctypes.util.find_library("fftw3f") else "not on the host)"),
+            "share_value": round(many.shape[0] * rows * W / dt_share / 1e6, 4), "share_cores": share,
+            "ssd_per_pixel_crop": round(ssd_crop, 2), "ssd_per_pixel_full_frame": round(ssd_full, 2)}
 
 
 def main():
@@ -350,6 +417,7 @@ def main():
     # work per launch, from one more pass with the frames one after another,
     # so that no other stage's kernels share the GPU with a timed launch
     work = {"k_match(basic)": [], "k_match(final)": [], "k_filter(hard)": [], "k_filter(wiener)": []}
+    executed = {"k_filter(hard)": [], "k_filter(wiener)": []}
     for c in (basic, final):
         c.filter.profile(True)
     for c in chain:
@@ -361,6 +429,7 @@ def main():
             ssd, dct = flops_per_frame(st, 8, wiener)
             work["k_match(final)" if wiener else "k_match(basic)"].append(ssd)
             work["k_filter(wiener)" if wiener else "k_filter(hard)"].append(dct)
+            executed["k_filter(wiener)" if wiener else "k_filter(hard)"].append(executed_filter_flops(st, 8, wiener))
     prof = {}
     for c in (basic, final):
         for name, (ms, n) in c.filter.profile_read().items():
@@ -392,13 +461,17 @@ def main():
         "config": {"workload": "RGB2OPP->Basic(np)->Final(np)->OPP2RGB, 8 frames 1920x1080 RGB48, sigma 10, "
                                "matrix=100, inputs resident in HBM",
                    "frames": FRAMES, "width": W, "height": H},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3) if achieved else None,
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3) if achieved else None,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4) if achieved else None,
                      "traffic": measured_traffic(dom),
-                     "frac_of_nonpacked_valu": round(achieved / PEAK_VALU_NONPACKED, 4) if achieved else None,
                      "avg_launch_ms": round(avg_ms, 4),
-                     "flops_per_launch": flops_launch},
+                     "flops_per_launch": flops_launch,
+                     "flops_count": "algorithmic: dense separable 3D DCT T*(4*B^3*n + 2*B^2*n^2) + c*B^2*n per "
+                                    "group and plane, from the frame's sum of n and n^2",
+                     "executed_flops_per_launch": (float(np.mean(executed[dom])) if executed.get(dom) else None),
+                     "executed_tflops": (round(float(np.mean(executed[dom])) / (avg_ms * 1e-3) / 1e12, 3)
+                                         if executed.get(dom) else None)},
         "kernels_ms_per_launch": {k: round(v[0], 4) for k, v in kern.items()},
     }
     if not args.no_learned:
@@ -411,7 +484,8 @@ def main():
         print("V-BM3D leg (30 frames 1080p YUV444 float)", file=sys.stderr, flush=True)
         result["vbm3d"] = vbm3d_leg()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(frames, args.cpu_rows, min(FRAMES, os.cpu_count() or 1))
+        cores = os.cpu_count() or 1
+        result["cpu_baseline"] = cpu_baseline(frames, args.cpu_rows, min(FRAMES, cores), min(16, cores))
     print(json.dumps(result), flush=True)
 
 

@@ -17,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libbm3d_amd.so")
+PLUGIN = os.path.join(OUT_DIR, "libbm3d_vs.so")   # the VapourSynth plugin shim over LIB
+CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BM3D_OFFLOAD_ARCH", "gfx950")
 
@@ -97,8 +99,26 @@ def build(verbose: bool = False) -> str:
         print(" ".join(shlex.quote(c) for c in cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_plugin(verbose)
     return LIB
 
 
+def build_plugin(verbose: bool = False) -> str:
+    """The VapourSynth API v4 plugin (host code only, no device code): loads
+    libbm3d_amd.so from its own directory."""
+    tmp = PLUGIN + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall", "-Wextra",
+           "-Wno-unused-parameter", os.path.join(HERE, "vs", "bm3d_vsplugin.cpp"), "-o", tmp, "-L", OUT_DIR,
+           "-lbm3d_amd", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    if verbose:
+        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, PLUGIN)
+    return PLUGIN
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    if "--plugin-only" in sys.argv:
+        print(build_plugin(verbose="-v" in sys.argv))
+    else:
+        print(build(verbose="-v" in sys.argv))
