@@ -795,24 +795,44 @@ __device__ __forceinline__ void dct8_inv(const float* in, float* out) {
     });
 }
 
-// x (rows) then y (columns) passes of the n slices in LDS (row stride 9).
+// Sixteen bytes of a row that starts at any float (dword alignment suffices
+// for global vector loads on this target).
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+// Gather and forward x pass of the N member blocks of one plane: lane takes
+// the lines L = (z, y), loads the member's row straight from the plane (two
+// 16-byte loads), transforms it in registers and writes the coefficients to
+// g[z][y][x] (row stride 9, so the z pass's column reads hit distinct banks).
+template <int N>
+__device__ __forceinline__ void rows_fwd(float* g, const float* const* planes, const int* zyx, int pitch,
+                                         bool single_frame) {
+    const int lane = lane_id();
+    constexpr int NL = N * 8;
+#pragma unroll
+    for (int L0 = 0; L0 < NL; L0 += 64) {
+        const int L = L0 + lane;
+        if (NL % 64 == 0 || L < NL) {
+            const int z = L >> 3, y = L & 7;
+            const float* base = single_frame ? planes[0] : planes[zyx[3 * z]];
+            const float* row = base + (size_t)(zyx[3 * z + 1] + y) * pitch + zyx[3 * z + 2];
+            const f4u a = *reinterpret_cast<const f4u*>(row);
+            const f4u b = *reinterpret_cast<const f4u*>(row + 4);
+            const float in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            float out[8];
+            dct8_fwd(in, out);
+            float* dst = g + L * 9;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dst[q] = out[q];
+        }
+    }
+    wave_sync_lds();
+}
+
+// y pass (forward or inverse) of the n slices in LDS: lines (z, x).
 template <bool FWD>
-__device__ __forceinline__ void passes_xy(float* g, int n) {
+__device__ __forceinline__ void pass_y(float* g, int n) {
     const int lane = lane_id();
     float in[8], out[8];
-    // x: lines (z, y)
-    if (FWD) {
-        for (int L = lane; L < n * 8; L += 64) {
-            float* row = g + L * 9;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) in[q] = row[q];
-            dct8_fwd(in, out);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) row[q] = out[q];
-        }
-        wave_sync_lds();
-    }
-    // y: lines (z, x)
     for (int L = lane; L < n * 8; L += 64) {
         float* col = g + (L >> 3) * 72 + (L & 7);
 #pragma unroll
@@ -823,17 +843,30 @@ __device__ __forceinline__ void passes_xy(float* g, int n) {
         for (int q = 0; q < 8; ++q) col[q * 9] = out[q];
     }
     wave_sync_lds();
-    if (!FWD) {
-        for (int L = lane; L < n * 8; L += 64) {
-            float* row = g + L * 9;
+}
+
+// Inverse x pass fused with the contribution stores: line L = (z, y) of the
+// group, inverse-transformed in registers, scaled by num_w and written as
+// contrib[z * 64 + y * 8 + x] (two aligned 16-byte stores per line).
+template <int N>
+__device__ __forceinline__ void rows_inv_store(const float* g, float* __restrict__ contrib, float num_w) {
+    const int lane = lane_id();
+    constexpr int NL = N * 8;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) in[q] = row[q];
+    for (int L0 = 0; L0 < NL; L0 += 64) {
+        const int L = L0 + lane;
+        if (NL % 64 == 0 || L < NL) {
+            const float* src = g + L * 9;
+            float in[8], out[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) in[q] = src[q];
             dct8_inv(in, out);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) row[q] = out[q];
+            float4* dst = reinterpret_cast<float4*>(contrib + L * 8);
+            dst[0] = float4{out[0] * num_w, out[1] * num_w, out[2] * num_w, out[3] * num_w};
+            dst[1] = float4{out[4] * num_w, out[5] * num_w, out[6] * num_w, out[7] * num_w};
         }
-        wave_sync_lds();
     }
+    wave_sync_lds();
 }
 
 // Forward z transform of the lane's column (y, x): g[z*72 + off] -> X[k].
@@ -911,28 +944,6 @@ __device__ __forceinline__ void zinvN(float* g, int off, const float* X) {
     });
 }
 
-// One group of N members, every processed plane (N compile time).
-// Loads the N member blocks of one plane into g[z][y][x] (row stride 9).
-template <int N>
-__device__ __forceinline__ void gather8(float* g, const float* const* planes, const int* zyx, int pitch, int off,
-                                        int y, int x, bool single_frame) {
-    if (single_frame) {
-        const float* base = planes[0];
-#pragma unroll
-        for (int z = 0; z < N; ++z) {
-            const int py = zyx[3 * z + 1], px = zyx[3 * z + 2];
-            g[z * 72 + off] = base[(size_t)(py + y) * pitch + px + x];
-        }
-    } else {
-#pragma unroll
-        for (int z = 0; z < N; ++z) {
-            const int pz = zyx[3 * z], py = zyx[3 * z + 1], px = zyx[3 * z + 2];
-            g[z * 72 + off] = planes[pz][(size_t)(py + y) * pitch + px + x];
-        }
-    }
-    wave_sync_lds();
-}
-
 // ONE: only plane 0 is processed (Gray, or YUV with the chroma planes
 // passed through), which keeps the other planes' pointers out of registers.
 template <int N, bool WIENER, bool ONE, bool SCALAR>
@@ -949,11 +960,11 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         float X[N];
         float den_w;
         if (!WIENER) {
-            gather8<N>(gs, srcp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
+            rows_fwd<N>(gs, srcp, zyx, a.p.pitch, a.frame == nullptr);
             float thr4[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) thr4[q] = tb.thr[N - 1][q];
-            passes_xy<true>(gs, N);
+            pass_y<true>(gs, N);
             zfwdN<N>(gs, off, X);
             int kept = 0;
 #pragma unroll
@@ -969,13 +980,13 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         } else {
             // one LDS buffer: the reference group's transform goes to registers
             // first, then the source group reuses the buffer
-            gather8<N>(gs, refp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
-            passes_xy<true>(gs, N);
+            rows_fwd<N>(gs, refp, zyx, a.p.pitch, a.frame == nullptr);
+            pass_y<true>(gs, N);
             float R[N];
             zfwdN<N>(gs, off, R);
             wave_sync_lds();
-            gather8<N>(gs, srcp, zyx, a.p.pitch, off, y, x, a.frame == nullptr);
-            passes_xy<true>(gs, N);
+            rows_fwd<N>(gs, srcp, zyx, a.p.pitch, a.frame == nullptr);
+            pass_y<true>(gs, N);
             zfwdN<N>(gs, off, X);
             wave_sync_lds();
             const float sig2 = tb.wiener_sigma_sqr[N - 1];
@@ -1037,13 +1048,10 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         }
         zinvN<N>(gs, off, X);
         wave_sync_lds();
-        passes_xy<false>(gs, N);
+        pass_y<false>(gs, N);
         const float num_w = (float)((double)den_w / tb.final_amp[N - 1]);
-        float* contrib = pick3(a.contrib, pl) + (size_t)g_id * G * 64;
-#pragma unroll
-        for (int z = 0; z < N; ++z) contrib[z * 64 + lane] = gs[z * 72 + off] * num_w;
+        rows_inv_store<N>(gs, pick3(a.contrib, pl) + (size_t)g_id * G * 64, num_w);
         if (lane == 0) pick3(a.denw, pl)[g_id] = den_w;
-        wave_sync_lds();
     }
 }
 
@@ -1080,7 +1088,7 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
 }
 
 template <int GMAX, bool WIENER, bool ONE, bool SCALAR>
-__global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
     const int slot = xcd_item(a.nb, 1, 0);
     if (slot < 0) return;
@@ -1093,7 +1101,7 @@ __global__ __launch_bounds__(64) void k_filter8(FilterArgs a) {
 // the batch-wide group number; ff[f] the frame's pitch, plane pointer table
 // (src p0..p2, ref p0..p2) and tables.  Plane 0 only (Gray frames).
 template <int GMAX, bool WIENER, bool SCALAR>
-__global__ __launch_bounds__(64) void k_filter8_batch(FilterArgs a0, const BatchFilterFrame* __restrict__ ff,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_filter8_batch(FilterArgs a0, const BatchFilterFrame* __restrict__ ff,
                                                       const int* __restrict__ starts, int nframes,
                                                       const int* __restrict__ order, int total) {
     extern __shared__ float fsm[];
