@@ -139,6 +139,28 @@ __device__ inline float ssd_generic(const float* rb, const float* __restrict__ q
 // sel_key/sel_idx[k..2k) by counting, for each entry, the entries before it.
 __device__ inline void rank_sort_selected(uint32_t* sel_key, int* sel_idx, int k) {
     const int lane = lane_id();
+    if (k <= 64) {
+        // one entry per lane
+        const uint32_t rk = lane < k ? sel_key[lane] : 0xFFFFFFFFu;
+        const int ri = lane < k ? sel_idx[lane] : 0x7FFFFFFF;
+        int rr = 0;
+        for (int e2 = 0; e2 < k; ++e2) {
+            const uint32_t k2 = sel_key[e2];
+            const int i2 = sel_idx[e2];
+            rr += (k2 < rk) || (k2 == rk && i2 < ri);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < k) {
+            sel_key[k + rr] = rk;
+            sel_idx[k + rr] = ri;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        return;
+    }
     uint32_t rk[4];
     int ri[4], rr[4];
 #pragma unroll

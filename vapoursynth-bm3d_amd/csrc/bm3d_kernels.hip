@@ -180,10 +180,14 @@ __global__ __launch_bounds__(64 * kMatchWaves) void k_match_list(const float* __
 constexpr int kSelCap = 256;
 
 // Reference block b of one plane (the body of k_match8 and its batched
-// form); wbase: the wave's LDS words.
+// form); wbase: the wave's wave_words LDS words.  Layout: the search window
+// (rows of (ncols + 7) | 1 samples, then W - 1 padding samples that the last
+// strips read past its last row) from word 0 and the reference block in the
+// last 64 words; once every SSD is computed, the selection scratch
+// (2 kSelCap + 4 G words) reuses the window's words.
 template <int KMAX, int W>
 __device__ __forceinline__ void match8_block(const float* __restrict__ plane, const StageParams& p, int b,
-                                             uint32_t* wbase, int* __restrict__ out_count,
+                                             uint32_t* wbase, int wave_words, int* __restrict__ out_count,
                                              int* __restrict__ out_pos, float* __restrict__ out_key) {
     const int lane = lane_id();
     constexpr int B = 8;
@@ -208,16 +212,25 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
     const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
     const int ncols = r - l + 1, nrows = bt - t + 1;
     const int nstr = (ncols + W - 1) / W;
-    const int WW = (nstr * W + 7) | 1;  // covers the last strip's W + 7 samples
+    const int WW = (ncols + 7) | 1;  // odd: lanes on consecutive rows read distinct banks
     const int WH = nrows + 7;
-    for (int e = lane; e < WH * WW; e += 64) {
-        int wy = e / WW, wx = e - wy * WW;
-        int gx = l + wx;
-        win[e] = gx < p.width ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
+    {
+        // element e = wy * WW + wx, advanced by 64 per step without a division
+        const int dq = 64 / WW, dr = 64 - dq * WW;
+        int wy = lane / WW, wx = lane - wy * WW;
+        for (int e = lane; e < WH * WW + W - 1; e += 64) {
+            const int gx = l + wx;
+            win[e] = (wy < WH && gx < p.width) ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
+            wx += dr;
+            wy += dq;
+            if (wx >= WW) {
+                wx -= WW;
+                ++wy;
+            }
+        }
     }
-    const int win_words = (WH * WW + 3) & ~3;
-    float* rbs = win + win_words;                                   // [64] reference block
-    uint32_t* s_key = wbase + win_words + 64;                       // [kSelCap]
+    float* rbs = win + wave_words - 64;                             // [64] reference block
+    uint32_t* s_key = wbase;                                        // [kSelCap], after the SSDs
     int* s_idx = reinterpret_cast<int*>(s_key + kSelCap);           // [kSelCap]
     uint32_t* sel_key = reinterpret_cast<uint32_t*>(s_idx + kSelCap);  // [2G]
     int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * G);         // [2G]
@@ -228,6 +241,7 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
 
     const int tasks = nrows * nstr;
     uint32_t keys[NK];
+    int cbase[KMAX];  // candidate index of the round's first strip element: cy * ncols + sx
     int naccepted = 0;
 #pragma unroll
     for (int tk = 0; tk < KMAX; ++tk) {
@@ -236,6 +250,7 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
         const int strip = active ? task / nrows : 0;
         const int cy = active ? task - strip * nrows : 0;
         const int sx = strip * W;
+        cbase[tk] = cy * ncols + sx;
         float s[W][4];
 #pragma unroll
         for (int k = 0; k < W; ++k)
@@ -277,11 +292,9 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
         }
     }
     // candidate index of register slot q = tk * W + k: cy * ncols + cx (row-major lattice order)
-    auto cand_idx = [&](int q) {
-        const int task = lane + 64 * (q / W);
-        const int strip = task / nrows;
-        return (task - strip * nrows) * ncols + strip * W + q % W;
-    };
+    auto cand_idx = [&](int q) { return cbase[q / W] + q % W; };
+    // the selection scratch reuses the window's words
+    wave_sync_lds();
     const int kk = naccepted < G - 1 ? naccepted : G - 1;
     const bool cut = naccepted > kk;
     int nsel = 0;  // entries written to sel_key/sel_idx[0..kk)
@@ -459,7 +472,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     (void)nlist;
     const int b = xcd_item(nb, NW, wave);
     if (b < 0) return;
-    match8_block<KMAX, W>(plane, p, b, smem + (size_t)wave * wave_words, out_count, out_pos, out_key);
+    match8_block<KMAX, W>(plane, p, b, smem + (size_t)wave * wave_words, wave_words, out_count, out_pos, out_key);
 }
 
 // Sparse re-run over the blocks blist[0 .. *nlist): a small grid whose waves
@@ -473,8 +486,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     const int wave = uniform(threadIdx.x >> 6);
     const int n = uniform(*nlist);
     for (int k = blockIdx.x * NW + wave; k < n && k < nb; k += gridDim.x * NW) {
-        match8_block<KMAX, W>(plane, p, uniform(blist[k]), smem + (size_t)wave * wave_words, out_count, out_pos,
-                              out_key);
+        match8_block<KMAX, W>(plane, p, uniform(blist[k]), smem + (size_t)wave * wave_words, wave_words,
+                              out_count, out_pos, out_key);
         wave_sync_lds();
     }
 }
@@ -516,8 +529,8 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     const StageParams p = frames[f].p;
     const float* plane = frames[f].plane;
     const int s0 = uniform(starts[f]);
-    match8_block<KMAX, W>(plane, p, gb - s0, smem + (size_t)wave * wave_words, count + s0, pos + (size_t)s0 * G,
-                          key + (size_t)s0 * G);
+    match8_block<KMAX, W>(plane, p, gb - s0, smem + (size_t)wave * wave_words, wave_words, count + s0,
+                          pos + (size_t)s0 * G, key + (size_t)s0 * G);
 }
 
 // ---------------------------------------------------------------------------
@@ -1463,14 +1476,17 @@ struct Match8Shape {
     size_t words;
 };
 
-// Shape of the B = 8 fast path for strip width w.
+// Shape of the B = 8 fast path for strip width w: rounds of 64 strips and
+// the wave's LDS words (match8_block's layout: the window with its padding,
+// or the selection scratch that later reuses it, and the reference block).
 static Match8Shape match8_shape_w(int max_cand, int G, int w) {
     const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
     const int nstr = (side + w - 1) / w;
     const int kmax = (side * nstr + 63) / 64;
-    const size_t ww = (size_t)((nstr * w + 7) | 1);
-    const size_t win = ((size_t)(side + 7) * ww + 3) & ~(size_t)3;
-    return {w, kmax, win + 64 + 2 * (size_t)kSelCap + 4 * (size_t)G};
+    const size_t ww = (size_t)((side + 7) | 1);
+    const size_t win = (size_t)(side + 7) * ww + (size_t)w - 1;
+    const size_t sel = 2 * (size_t)kSelCap + 4 * (size_t)G;
+    return {w, kmax, (((win > sel ? win : sel) + 3) & ~(size_t)3) + 64};
 }
 
 Match8Shape match8_shape(int max_cand, int G) {
@@ -1483,9 +1499,7 @@ Match8Shape match8_shape(int max_cand, int G) {
         const long slots = (long)kmax * 64 * w;
         if (best_slots < 0 || slots < best_slots) {
             best_slots = slots;
-            const size_t ww = (size_t)((nstr * w + 7) | 1);
-            const size_t win = ((size_t)(side + 7) * ww + 3) & ~(size_t)3;
-            best = {w, kmax, win + 64 + 2 * (size_t)kSelCap + 4 * (size_t)G};
+            best = match8_shape_w(max_cand, G, w);
         }
     }
     return best;
