@@ -33,7 +33,7 @@ typedef struct bm3d_batch bm3d_batch;
 /* Launch shape of the batched stages (chosen by bm3d_batch_tune). */
 typedef struct bm3d_batch_shape {
     int match_waves;  /* reference blocks (waves) per matching workgroup: 1, 2 or 4 */
-    int match_strip;  /* candidates per lane strip of the LDS search tile: 8 or 11 */
+    int match_strip;  /* candidates per lane strip of the LDS search tile: 3, 8, 10 or 11 */
     int group_order;  /* 0: groups bucketed by size per frame; 1: per chunk of 4096 groups */
 } bm3d_batch_shape;
 

@@ -373,10 +373,10 @@ int bm3d_batch_create(bm3d_batch** out, int nframes, const int* widths, const in
         s.ag.resize(nframes);
         s.ptrs.assign(6 * (size_t)nframes, nullptr);
     }
-    // default launch shape: 2 waves per matching workgroup, the wider strip
-    // when it fits, size buckets per chunk of groups
+    // default launch shape: 2 waves per matching workgroup, the first strip
+    // width of 3, 11, 8 that fits, size buckets per chunk of groups
     b->shape = BatchShape{2, 11, kOrderChunk};
-    for (int w : {11, 8}) {
+    for (int w : {3, 11, 8}) {
         BatchShape t{2, w, kOrderChunk};
         bool ok = true;
         for (const Stage& s : b->st)
@@ -499,7 +499,7 @@ int bm3d_batch_tune(bm3d_batch* b, const float* const* src, float* const* dst, i
     if (any_batched) {
         // matching shape first (at the default group order), then the order
         for (int nw : {1, 2, 4})
-            for (int w : {8, 11}) {
+            for (int w : {3, 8, 10, 11}) {
                 BatchShape t{nw, w, best.group_order};
                 if (!shape_ok(b, t)) continue;
                 float ms;

@@ -84,7 +84,7 @@ enum { kOrderFrame = 0, kOrderChunk = 1 };
 // Tunable launch shapes of the batched stages.
 struct BatchShape {
     int match_waves;   // reference blocks (waves) per matching workgroup: 1, 2 or 4
-    int match_strip;   // candidates per lane strip in the LDS search tile: 8 or 11
+    int match_strip;   // candidates per lane strip in the LDS search tile: 3, 8, 10 or 11
     int group_order;   // kOrderFrame: size buckets per frame; kOrderChunk: per chunk of groups
 };
 

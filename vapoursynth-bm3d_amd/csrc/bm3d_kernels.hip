@@ -215,17 +215,29 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
     const int WW = (ncols + 7) | 1;  // odd: lanes on consecutive rows read distinct banks
     const int WH = nrows + 7;
     {
-        // element e = wy * WW + wx, advanced by 64 per step without a division
+        // element e = wy * WW + wx, advanced by 64 per step without a
+        // division; eight steps' loads in flight before their LDS writes
         const int dq = 64 / WW, dr = 64 - dq * WW;
+        const int n = WH * WW + W - 1;
         int wy = lane / WW, wx = lane - wy * WW;
-        for (int e = lane; e < WH * WW + W - 1; e += 64) {
-            const int gx = l + wx;
-            win[e] = (wy < WH && gx < p.width) ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
-            wx += dr;
-            wy += dq;
-            if (wx >= WW) {
-                wx -= WW;
-                ++wy;
+        for (int e0 = 0; e0 < n; e0 += 8 * 64) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 64 * u + lane;
+                const int gx = l + wx;
+                v[u] = (e < n && wy < WH && gx < p.width) ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
+                wx += dr;
+                wy += dq;
+                if (wx >= WW) {
+                    wx -= WW;
+                    ++wy;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 64 * u + lane;
+                if (e < n) win[e] = v[u];
             }
         }
     }
@@ -1476,30 +1488,46 @@ struct Match8Shape {
     size_t words;
 };
 
+// (rounds, strip width) pairs the 8x8 matcher is built for: X(K, W, NW).
+#define BM3D_M8_SHAPES(X, NW)                                                                  \
+    X(1, 8, NW) X(2, 8, NW) X(3, 8, NW) X(4, 8, NW) X(1, 11, NW) X(2, 11, NW) X(3, 11, NW)     \
+    X(4, 11, NW) X(2, 3, NW) X(4, 3, NW) X(6, 3, NW) X(3, 10, NW) X(4, 10, NW)
+static const int kM8Shapes[][2] = {{1, 8},  {2, 8}, {3, 8}, {4, 8}, {1, 11}, {2, 11}, {3, 11},
+                                   {4, 11}, {2, 3}, {4, 3}, {6, 3}, {3, 10}, {4, 10}};
+
+// The fewest built rounds of strip width w that cover kmax rounds (0: none).
+static int m8_rounds(int w, int kmax) {
+    int best = 0;
+    for (const auto& s : kM8Shapes)
+        if (s[1] == w && s[0] >= kmax && (best == 0 || s[0] < best)) best = s[0];
+    return best;
+}
+
 // Shape of the B = 8 fast path for strip width w: rounds of 64 strips and
 // the wave's LDS words (match8_block's layout: the window with its padding,
 // or the selection scratch that later reuses it, and the reference block).
 static Match8Shape match8_shape_w(int max_cand, int G, int w) {
     const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
     const int nstr = (side + w - 1) / w;
-    const int kmax = (side * nstr + 63) / 64;
+    const int kmax = m8_rounds(w, (side * nstr + 63) / 64);
     const size_t ww = (size_t)((side + 7) | 1);
     const size_t win = (size_t)(side + 7) * ww + (size_t)w - 1;
     const size_t sel = 2 * (size_t)kSelCap + 4 * (size_t)G;
     return {w, kmax, (((win > sel ? win : sel) + 3) & ~(size_t)3) + 64};
 }
 
+// The built shape with the fewest strip slots (rounds x width: every round
+// runs whole); among equals the wider strip (fewer window reads).
 Match8Shape match8_shape(int max_cand, int G) {
-    const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
     Match8Shape best{0, 0, 0};
     long best_slots = -1;
-    for (int w : {8, 11}) {
-        const int nstr = (side + w - 1) / w;
-        const int kmax = (side * nstr + 63) / 64;
-        const long slots = (long)kmax * 64 * w;
+    for (int w : {11, 10, 8, 3}) {
+        const Match8Shape sh = match8_shape_w(max_cand, G, w);
+        if (sh.kmax == 0) continue;
+        const long slots = (long)sh.kmax * w;
         if (best_slots < 0 || slots < best_slots) {
             best_slots = slots;
-            best = match8_shape_w(max_cand, G, w);
+            best = sh;
         }
     }
     return best;
@@ -1512,7 +1540,7 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
     // the scalar flavour runs the generic matcher
     if (p.block == 8 && p.step == 1 && p.arith == kArithSSE2) {
         const Match8Shape sh = match8_shape(max_cand, p.group);
-        const int km = sh.kmax <= 1 ? 1 : sh.kmax <= 2 ? 2 : sh.kmax <= 3 ? 3 : sh.kmax <= 4 ? 4 : 0;
+        const int km = sh.kmax;
         const size_t wbytes = sh.words * 4;
         const int nw = wbytes * 4 <= 65536 ? 4 : wbytes * 2 <= 65536 ? 2 : wbytes <= 65536 ? 1 : 0;
         if (km != 0 && nw != 0) {
@@ -1530,13 +1558,9 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
                                (int)sh.words, blist, nlist, count, pos, key);                                \
         return hipGetLastError();                                                                            \
     }
-#define BM3D_M8W(NW) \
-    BM3D_M8(1, 8, NW) BM3D_M8(2, 8, NW) BM3D_M8(3, 8, NW) BM3D_M8(4, 8, NW) \
-    BM3D_M8(1, 11, NW) BM3D_M8(2, 11, NW) BM3D_M8(3, 11, NW) BM3D_M8(4, 11, NW)
-            BM3D_M8W(4)
-            BM3D_M8W(2)
-            BM3D_M8W(1)
-#undef BM3D_M8W
+            BM3D_M8_SHAPES(BM3D_M8, 4)
+            BM3D_M8_SHAPES(BM3D_M8, 2)
+            BM3D_M8_SHAPES(BM3D_M8, 1)
 #undef BM3D_M8
         }
     }
@@ -1662,10 +1686,10 @@ int device_cus() {
 
 bool batch_match_supported(const StageParams& p, int max_cand, const BatchShape& sh) {
     if (p.block != 8 || p.step != 1 || p.arith != kArithSSE2) return false;
-    if (sh.match_strip != 8 && sh.match_strip != 11) return false;
+    if (sh.match_strip != 3 && sh.match_strip != 8 && sh.match_strip != 10 && sh.match_strip != 11) return false;
     if (sh.match_waves != 1 && sh.match_waves != 2 && sh.match_waves != 4) return false;
     const Match8Shape ms = match8_shape_w(max_cand, p.group, sh.match_strip);
-    return ms.kmax >= 1 && ms.kmax <= 4 && ms.words * 4 * sh.match_waves <= 65536;
+    return ms.kmax >= 1 && ms.words * 4 * sh.match_waves <= 65536;
 }
 
 // Workgroups of a batched launch (xcd_item): one per `per` items, rounded up
@@ -1681,7 +1705,7 @@ hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int n
     if (total <= 0) return hipSuccess;
     if (!batch_match_supported(p0, max_cand, sh)) return hipErrorInvalidValue;
     const Match8Shape ms = match8_shape_w(max_cand, p0.group, sh.match_strip);
-    const int km = ms.kmax <= 1 ? 1 : ms.kmax <= 2 ? 2 : ms.kmax <= 3 ? 3 : 4;
+    const int km = ms.kmax;
     const int nw = sh.match_waves;
     const size_t lds = ms.words * 4 * nw;
     const int groups = batch_groups(total, nw);
@@ -1691,13 +1715,9 @@ hipError_t launch_match_batch(const BatchFrame* frames, const int* starts, int n
                            nframes, total, (int)ms.words, p0.group, count, pos, key);                        \
         return hipGetLastError();                                                                            \
     }
-#define BM3D_M8BW(NW) \
-    BM3D_M8B(1, 8, NW) BM3D_M8B(2, 8, NW) BM3D_M8B(3, 8, NW) BM3D_M8B(4, 8, NW) \
-    BM3D_M8B(1, 11, NW) BM3D_M8B(2, 11, NW) BM3D_M8B(3, 11, NW) BM3D_M8B(4, 11, NW)
-    BM3D_M8BW(1)
-    BM3D_M8BW(2)
-    BM3D_M8BW(4)
-#undef BM3D_M8BW
+    BM3D_M8_SHAPES(BM3D_M8B, 1)
+    BM3D_M8_SHAPES(BM3D_M8B, 2)
+    BM3D_M8_SHAPES(BM3D_M8B, 4)
 #undef BM3D_M8B
     return hipErrorInvalidValue;
 }
