@@ -68,37 +68,44 @@ def _basic(bm3d, img, **kw):
 KW = dict(profile="fast", sigma=25.0, bm_range=12, group_size=16)
 
 
-def _exact_key(img, j, i, y, x):
+def _exact_key(img, j, i, y, x, mode=O.ARITH_SSE2):
     rb = np.ascontiguousarray(img[j:j + 8, i:i + 8])
-    d = O.lib().or_ssd(O._fp(rb), O._fp(np.ascontiguousarray(img)), img.shape[1], y, x, 8, O.ARITH_SSE2)
+    d = O.lib().or_ssd(O._fp(rb), O._fp(np.ascontiguousarray(img)), img.shape[1], y, x, 8, mode)
     return np.float32(np.float64(d) * DIST_MUL)
 
 
-def test_learned_lists_are_exact_reranks(gpu):
+@pytest.mark.parametrize("arith", [0, 1])
+def test_learned_lists_are_exact_reranks(gpu, arith):
     h, w = 96, 128
     noisy, _ = _plane(h, w, 25.0, 902)
     model = lbm.Model(seed=7)  # untrained: proposals are poor, the re-rank must still be exact
-    node = _basic(gpu, noisy, **KW)
+    node = _basic(gpu, noisy, arith=arith, **KW)
     node.filter.set_matcher(model, 32)
     node.get_frame(0)
     cnt, pos, key = node.filter.match_lists()
+    mode = O.ARITH_SCALAR if arith else O.ARITH_SSE2
     for b in range(0, len(cnt), 5):
         n = int(cnt[b])
         j, i = pos[b, 0]
         assert key[b, 0] == 0.0
         ks = key[b, 1:n]
         assert np.all(np.diff(ks) >= 0), b
+        assert len({(int(y), int(x)) for y, x in pos[b, :n]}) == n, ("duplicate position", b)
         for z in range(1, n):
             y, x = pos[b, z]
             assert (y, x) != (j, i)
             assert abs(int(y) - int(j)) <= 12 and abs(int(x) - int(i)) <= 12
-            assert key[b, z] == _exact_key(noisy, j, i, y, x), (b, z)
+            assert key[b, z] == _exact_key(noisy, j, i, y, x, mode), (b, z)
 
 
-def test_whole_window_proposals_equal_plain(gpu):
+@pytest.mark.parametrize("bm_range", [3, 4])
+def test_whole_window_proposals_equal_plain(gpu, bm_range):
+    # windows of at most 9x9 positions hold at most 25 lattice points, all of
+    # them proposed (top-k 64 -> 32 proposals): the 3x3 neighbourhoods then
+    # cover every window position exactly once
     h, w = 64, 96
     noisy, _ = _plane(h, w, 25.0, 903)
-    kw = dict(profile="fast", sigma=25.0, bm_range=3, group_size=16)  # 48 candidates <= top-k 64
+    kw = dict(profile="fast", sigma=25.0, bm_range=bm_range, group_size=16)
     plain = _basic(gpu, noisy, **kw)
     out_p = np.asarray(plain.get_frame(0).planes[0])
     learned = _basic(gpu, noisy, **kw)

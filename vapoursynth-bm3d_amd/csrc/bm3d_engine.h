@@ -57,7 +57,7 @@ struct DevBuf {
 
 // Work buffers of the learned proposal path of one filter.
 struct LbmWork {
-    DevBuf emb, norm, remb, rnorm, fallback, nfallback, bits;
+    DevBuf emb, norm, remb, rnorm, fallback, nfallback, props;
 };
 
 // Where one frame call takes its device scratch from.  A null Scratch*

@@ -129,7 +129,7 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     if ((rc = w.rnorm.ensure((size_t)nb * sizeof(float)))) return rc;
     if ((rc = w.fallback.ensure((size_t)nb * sizeof(int)))) return rc;
     if ((rc = w.nfallback.ensure(sizeof(int)))) return rc;
-    if ((rc = w.bits.ensure(L::propose_bits_words(sp) * sizeof(uint32_t)))) return rc;
+    if ((rc = w.props.ensure(L::propose_list_bytes(sp)))) return rc;
     const L::Net n = m->net();
     HIP_TRY(L::launch_embed(n, plane, sp.pitch, W, H, w.emb.as<_Float16>(), w.norm.as<float>(), st));
     HIP_TRY(L::launch_embed_refs(n, plane, sp, w.remb.as<_Float16>(), w.rnorm.as<float>(), st));
@@ -149,7 +149,7 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     pa.key = key;
     pa.fallback = w.fallback.as<int>();
     pa.nfallback = w.nfallback.as<int>();
-    pa.cand_bits = w.bits.as<uint32_t>();
+    pa.props = w.props.as<uint16_t>();
     HIP_TRY(L::launch_propose(pa, st));
     HIP_TRY(launch_match(plane, sp, nb, max_cand, count, pos, key, st, w.fallback.as<int>(), w.nfallback.as<int>()));
     return BM3D_OK;

@@ -23,6 +23,7 @@ constexpr int kParams = kOffB2 + kEmb;
 constexpr int kMaxProposals = 64;   // largest top-K
 constexpr int kLat = 2;             // proposal lattice stride (rows and columns)
 constexpr int kMaxWindow = 49 * 49; // largest search window (bm_range 24)
+constexpr int kPropSlots = kMaxProposals / 2;  // lattice points proposed per reference block (at most)
 
 // Device view of one network: FP32 master parameters and the BF16 copies of
 // the two weight matrices the MFMA kernels read.
@@ -64,7 +65,7 @@ struct ProposeArgs {
     float* key;
     int* fallback;   // reference blocks left to the exhaustive search
     int* nfallback;  // their number (device counter, cleared by the caller)
-    uint32_t* cand_bits;  // [nx * ny][propose_bits_words / (nx * ny)] candidate bitmaps
+    uint16_t* props;      // [nx * ny][kPropSlots] proposed lattice points (block-local index, 0xFFFF = none)
 };
 
 hipError_t launch_train_fwd(const Net& n, const TrainBatch& tb, hipStream_t st);
@@ -82,7 +83,7 @@ hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams
                              hipStream_t st);
 constexpr int kMaxLearnedGroupSize = 64;
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
-size_t propose_bits_words(const StageParams& p);
+size_t propose_list_bytes(const StageParams& p);
 
 }  // namespace lbm
 }  // namespace bm3d

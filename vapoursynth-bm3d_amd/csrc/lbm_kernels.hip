@@ -470,25 +470,179 @@ __device__ __forceinline__ void embed_groups(const Net& n, const float* __restri
     }
 }
 
+// The lattice embeddings run the network on v_mfma_f32_16x16x32_bf16 (half
+// the instructions of the 16x16x16 form; same bf16 operands and roundings of
+// the activations, FP32 accumulation in the instruction's own order).  A k-step
+// pairs lane group q's eight values of A and B, so the contraction index can be
+// assigned freely as long as both operands use the same assignment:
+//   layer 1, k-step s: lane group q holds pixel row 2q + s (eight columns);
+//   layer 2, k-step s: lane group q holds hidden units 16 (2s + j / 4) + 4q +
+//   j % 4, j = 0..7 -- exactly the layer-1 outputs the lane already holds.
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f4 mfma32(s8 a, s8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a), __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
+
+struct Frags32 {
+    s8 w1[8][2];  // layer 1: tile t (hidden 16t..), k-step s
+    s8 w2[2][4];  // layer 2: tile u (embedding 16u..), k-step s
+    float b1[8][4];
+    float b2[2][4];
+};
+
+__device__ __forceinline__ void load_frags32(const Net& n, Frags32& f) {
+    const int lane = lane_id();
+    const int r = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) __builtin_memcpy(&f.w1[t][s], n.w1h + (16 * t + r) * kIn + (2 * q + s) * 8, 16);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const short* row = n.w2h + (16 * u + r) * kHidden;
+            s4 lo, hi;
+            __builtin_memcpy(&lo, row + 16 * (2 * s) + 4 * q, 8);
+            __builtin_memcpy(&hi, row + 16 * (2 * s + 1) + 4 * q, 8);
+            f.w2[u][s] = s8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        }
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f.b1[t][i] = n.b1[16 * t + 4 * q + i];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f.b2[u][i] = n.b2[16 * u + 4 * q + i];
+}
+
+// The network on 16 patches: x[s] = the lane's pixel row 2q + s (bf16).
+__device__ __forceinline__ void infer32(const Frags32& f, const s8 (&x)[2], float (&e)[2][4]) {
+    s4 hb[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f4 acc = {f.b1[t][0], f.b1[t][1], f.b1[t][2], f.b1[t][3]};
+        acc = mfma32(f.w1[t][0], x[0], acc);
+        acc = mfma32(f.w1[t][1], x[1], acc);
+        const sh2 lo = relu_bf16x2(acc[0], acc[1]), hi = relu_bf16x2(acc[2], acc[3]);
+        hb[t] = s4{lo[0], lo[1], hi[0], hi[1]};
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        f4 acc = {f.b2[u][0], f.b2[u][1], f.b2[u][2], f.b2[u][3]};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const s4 a = hb[2 * s], c = hb[2 * s + 1];
+            acc = mfma32(f.w2[u][s], s8{a[0], a[1], a[2], a[3], c[0], c[1], c[2], c[3]}, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) e[u][i] = acc[i];
+    }
+}
+
+// four pixels -> bf16 (round to nearest even, packed conversion)
+__device__ __forceinline__ s4 frag_cvt(const float4& v) {
+    const fl2 a = {v.x, v.y}, b = {v.z, v.w};
+    const sh2 lo = __builtin_bit_cast(sh2, __builtin_convertvector(a, bf2));
+    const sh2 hi = __builtin_bit_cast(sh2, __builtin_convertvector(b, bf2));
+    return s4{lo[0], lo[1], hi[0], hi[1]};
+}
+
+// eight pixels (two float4) -> bf16
+__device__ __forceinline__ s8 row_cvt(const float4& a, const float4& b) {
+    const s4 lo = frag_cvt(a), hi = frag_cvt(b);
+    return s8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
 // Embeddings of the lattice: patch (kLat yy, kLat xx) -> index yy * ew2 + xx,
-// ew2 = ceil((W - 7) / kLat), for yy < ceil((H - 7) / kLat).
+// ew2 = ceil((W - 7) / kLat), for yy < ceil((H - 7) / kLat).  One wave per
+// column of 16 lattice positions and run of kEmbedRun lattice rows.  Lane
+// group q holds pixel rows 2q and 2q + 1 of its patch, so one lattice row
+// further down (kLat = 2 pixel rows) its rows are those group q + 1 held: a
+// 16-lane shift, and group 3 loads the two new rows, issued kEmbedAhead
+// lattice rows ahead.
+constexpr int kEmbedRun = 16;
+constexpr int kEmbedAhead = 4;
+static_assert(kLat == 2, "row reuse between lattice rows assumes a stride of two rows");
+
 __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __restrict__ plane, int pitch, int W, int H,
                                                        _Float16* __restrict__ emb, float* __restrict__ norm) {
     const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
-    const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
+    const int gx = (ew2 + 15) / 16;
+    const int runs = (eh2 + kEmbedRun - 1) / kEmbedRun;
     const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    if (wave >= per_row * eh2) return;
-    const int yy = wave / per_row;
-    const int xs = (wave - yy * per_row) * 16 * kEmbedGroups;
-    embed_groups(
-        n, plane, pitch,
-        [&](int g, int r, int& y, int& x) {
-            const int xx = xs + 16 * g + r;
-            y = kLat * yy;
-            x = kLat * (xx < ew2 ? xx : ew2 - 1);
-            return xx < ew2 ? yy * ew2 + xx : -1;
-        },
-        emb, norm);
+    if (wave >= gx * runs) return;
+    const int ry = wave / gx;
+    const int lane = lane_id();
+    const int r = lane & 15, q = lane >> 4;
+    const int xx = (wave - ry * gx) * 16 + r;
+    const bool xok = xx < ew2;
+    const float* col = plane + kLat * (xok ? xx : ew2 - 1);
+    const int yy0 = ry * kEmbedRun;
+    const int nrows = eh2 - yy0 < kEmbedRun ? eh2 - yy0 : kEmbedRun;
+    const int y0 = kLat * yy0;
+    // first pixel row group 3 loads for lattice row yy0 + k (k >= 1), clamped
+    // for rows past the lattice (loaded, never used)
+    auto new_row = [&](int k) {
+        const int y = y0 + kLat * k + 6;
+        return y < H - 1 ? y : H - 2;
+    };
+    Frags32 f;
+    load_frags32(n, f);
+    s8 x[2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        float4 a, b;
+        const float* src = col + (size_t)(y0 + 2 * q + s) * pitch;
+        __builtin_memcpy(&a, src, sizeof(a));
+        __builtin_memcpy(&b, src + 4, sizeof(b));
+        x[s] = row_cvt(a, b);
+    }
+    // (every lane loads the new rows, so the loads stay outside branches and
+    // in flight; group 3 uses them)
+    float4 ahead[kEmbedAhead][4];
+#pragma unroll
+    for (int k = 0; k < kEmbedAhead; ++k) {
+            const float* src = col + (size_t)new_row(k + 1) * pitch;
+            __builtin_memcpy(&ahead[k][0], src, 16);
+            __builtin_memcpy(&ahead[k][1], src + 4, 16);
+            __builtin_memcpy(&ahead[k][2], src + pitch, 16);
+            __builtin_memcpy(&ahead[k][3], src + pitch + 4, 16);
+        }
+    // every row of the run is evaluated (rows past the lattice store nothing):
+    // no exit inside the unrolled body keeps the loads in flight across rows
+#pragma unroll
+    for (int k = 0; k < kEmbedRun; ++k) {
+        float e[2][4];
+        infer32(f, x, e);
+        store_emb(e, xok && k < nrows ? (yy0 + k) * ew2 + xx : -1, emb, norm);
+        if (k + 1 == kEmbedRun) break;
+        // next lattice row: group q takes group q + 1's rows, group 3 the new ones
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            s8 v;
+#pragma unroll
+            for (int i = 0; i < 8; i += 2) {
+                const int w = __shfl_down(__builtin_bit_cast(int, sh2{x[s][i], x[s][i + 1]}), 16);
+                const sh2 p2 = __builtin_bit_cast(sh2, w);
+                v[i] = p2[0];
+                v[i + 1] = p2[1];
+            }
+            const s8 nv = row_cvt(ahead[k % kEmbedAhead][2 * s], ahead[k % kEmbedAhead][2 * s + 1]);
+            x[s] = q == 3 ? nv : v;
+        }
+        if (k + 1 + kEmbedAhead < kEmbedRun) {
+            const float* src = col + (size_t)new_row(k + 1 + kEmbedAhead) * pitch;
+            float4 (&d)[4] = ahead[k % kEmbedAhead];
+            __builtin_memcpy(&d[0], src, 16);
+            __builtin_memcpy(&d[1], src + 4, 16);
+            __builtin_memcpy(&d[2], src + pitch, 16);
+            __builtin_memcpy(&d[3], src + pitch + 4, 16);
+        }
+    }
 }
 
 // Embeddings of the reference blocks (scan order b = by * nx + bx).
@@ -515,72 +669,39 @@ __global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restri
 // inference: proposals, exact re-rank and fallback list
 // ---------------------------------------------------------------------------
 //
-// One workgroup (four waves) per tile of kPropRows x kPropCols reference
-// blocks.  Embedding distance: score(r, c) = |e_r|^2 + |e_c|^2 - 2 e_r.e_c
-// (fp32; fp16 in the per-block lists).  The dot products of the 16 blocks of a
-// tile row with 16 lattice positions of a union row are one 16x16x32 product
-// (two v_mfma_f32_16x16x16_f16).  Tiles are dealt to the XCDs in column-major
-// runs, so vertically adjacent tiles, whose unions overlap most, share an L2.
+// k_propose: one workgroup (four waves) per tile of kPropRows x kPropCols = 8
+// x 2 reference blocks.  Embedding distance: score(r, c) = |e_r|^2 + |e_c|^2 -
+// 2 e_r.e_c (fp32; fp16 in the per-block lists).  The dot products of the 16
+// blocks with 16 lattice positions of one row of the union of their windows
+// are one v_mfma_f32_16x16x32_f16 (a step); tall tiles keep the union small
+// (at bm_range 24 and block_step 8: 53 x 29 lattice positions, 106 steps for
+// 16 x 625 scores).  Each wave keeps kPropDepth steps' embedding loads in
+// flight.  Tiles are dealt to the XCDs in contiguous runs (column-major, so
+// vertically adjacent tiles, whose unions overlap most, share an L2).  Every
+// even-lattice position (even row, even column) of each block's search window
+// is scored, and each block proposes its M = min(K, 32) best lattice
+// positions (ties to the earlier position) as a list of block-local lattice
+// indices.
 //
-// Scoring (per tile row): every even-lattice position (even row, even column)
-// of each block's search window is scored.  Each block proposes its M = K/2
-// best lattice positions (ties to the earlier position); the candidates are
-// the 3x3 neighbourhoods of the proposals inside the window, self excluded,
-// marked in a per-block bitmap, so they come out in window scan order.
-//
-// Per block (one wave): every candidate is re-ranked exactly like the plain
-// matcher (SSE2 lane-order SSD, dist <= thSSE && dist != 0, key =
-// float(double(dist) * distMul)) and the G-1 best by (key, window index)
-// follow the self entry.  A block with fewer than G-1 accepted candidates
-// falls back to the exhaustive search; a window of at most K candidates is
-// searched exhaustively here.
-constexpr int kPropCols = 16;
-constexpr int kPropRows = 4;
+// k_rerank: the candidates of a block are the 3x3 neighbourhoods of its
+// proposals inside the window, self excluded; a position covered by several
+// proposals belongs to the first of them in lattice order.  Every candidate
+// is re-ranked exactly like the plain matcher (SSE2 lane-order SSD, dist <=
+// thSSE && dist != 0, key = float(double(dist) * distMul)) and the G-1 best
+// by (key, window index) follow the self entry.  A block with fewer than G-1
+// accepted candidates falls back to the exhaustive search, unless every
+// lattice position of its window was proposed: the neighbourhoods then cover
+// the whole window and the list is the exhaustive one.
+constexpr int kPropRows = 8;
+constexpr int kPropCols = 2;
+constexpr int kPropDepth = 4;  // scoring steps in flight per wave
 constexpr int kSubSide = (49 + kLat - 1) / kLat;  // lattice points per side of the largest window
-constexpr int kSubCap = kSubSide * kSubSide;  // lattice scores per block of a tile row
-constexpr int kListCap = 9 * (kMaxProposals / 2);
-constexpr int kBitWords = (kMaxWindow + 31) / 32;
+constexpr int kSubCap = kSubSide * kSubSide;  // lattice scores of one block
+static_assert(kPropRows * kPropCols == 16, "one MFMA row per block of a tile");
 
-__device__ __forceinline__ f4 mfma_h(h4 a, h4 b, f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
-
-// SSD of the 8x8 block rb (row-major registers) against the block at q, in
-// the SSE2 lane order of ssd_fixed<8> (scalar: the row-major order of builds
-// without __SSE2__, Block.h:632-643).
-template <bool SCALAR>
-__device__ __forceinline__ float ssd8_rows(const float* rb, const float* __restrict__ q, int pitch) {
-    if constexpr (SCALAR) {
-        float dist = 0.0f;
-#pragma unroll
-        for (int r = 0; r < 8; ++r)
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                const float t = rb[r * 8 + c] - q[r * pitch + c];
-                dist = dist + t * t;
-            }
-        return dist;
-    }
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-        float4 lo, hi;
-        __builtin_memcpy(&lo, q + r * pitch, sizeof(lo));
-        __builtin_memcpy(&hi, q + r * pitch + 4, sizeof(hi));
-        const float d0 = rb[r * 8 + 0] - lo.x, d1 = rb[r * 8 + 1] - lo.y;
-        const float d2 = rb[r * 8 + 2] - lo.z, d3 = rb[r * 8 + 3] - lo.w;
-        s0 = s0 + d0 * d0;
-        s1 = s1 + d1 * d1;
-        s2 = s2 + d2 * d2;
-        s3 = s3 + d3 * d3;
-        const float e0 = rb[r * 8 + 4] - hi.x, e1 = rb[r * 8 + 5] - hi.y;
-        const float e2 = rb[r * 8 + 6] - hi.z, e3 = rb[r * 8 + 7] - hi.w;
-        s0 = s0 + e0 * e0;
-        s1 = s1 + e1 * e1;
-        s2 = s2 + e2 * e2;
-        s3 = s3 + e3 * e3;
-    }
-    return 0.0f + (((s0 + s1) + s2) + s3);
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4 mfma_h32(h8 a, h8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // Marks v[u] (16-bit values, 0xFFFF = absent; value index lane + 64 u) that
@@ -612,16 +733,39 @@ __device__ __forceinline__ uint32_t wave_smallest16(const uint32_t (&v)[N], int 
     return sel;
 }
 
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Search window of block (by, bx) on the lattice: first lattice row/column
+// (plane coordinates) and the number of lattice rows/columns.
+struct LatWin {
+    int y0, ny, x0, nx;
+};
+__device__ __forceinline__ LatWin lat_window(const StageParams& p, int by, int bx) {
+    const int B = 8;
+    const int j = grid_pos(by, p.block_step, p.height - B);
+    const int i = grid_pos(bx, p.block_step, p.width - B);
+    const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
+    const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
+    LatWin w;
+    w.y0 = lat_first(t);
+    w.ny = lat_span(w.y0, bt);
+    w.x0 = lat_first(l);
+    w.nx = lat_span(w.x0, rr);
+    return w;
+}
 
 __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
-    __shared__ unsigned short sub[kPropCols][kSubCap];  // lattice scores of one tile row
-    __shared__ uint32_t bits[4][kBitWords];             // candidates of the block each wave marks
-    __shared__ uint32_t comp[4][128];                   // per wave: compacted lattice scores and positions
+    __shared__ unsigned short sub[16][kSubCap];  // lattice scores of the tile's blocks
+    __shared__ uint32_t comp[4][128];            // per wave: compacted lattice scores and positions
     const StageParams& p = a.p;
     const int lane = lane_id();
     const int wave = uniform((int)(threadIdx.x >> 6));
     const int r = lane & 15, q = lane >> 4;
-    // tile of the workgroup (XCD-contiguous, column-major)
+    const int G = p.group;
+    if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) return;  // k_rerank writes the self-only lists
+    // tile of the workgroup (XCD-contiguous runs, column-major)
     const int row_tiles = (p.ny + kPropRows - 1) / kPropRows;
     const int col_tiles = (p.nx + kPropCols - 1) / kPropCols;
     const int ntiles = row_tiles * col_tiles;
@@ -629,218 +773,235 @@ __global__ __launch_bounds__(256) void k_propose(ProposeArgs a) {
     const int lin = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
     if (lin >= ntiles) return;
     const int tcol = lin / row_tiles;
-    const int bj0 = (lin - tcol * row_tiles) * kPropRows;
-    const int bi0 = tcol * kPropCols;
+    const int by0 = (lin - tcol * row_tiles) * kPropRows;
+    const int bx0 = tcol * kPropCols;
+    const int by_last = by0 + kPropRows - 1 < p.ny - 1 ? by0 + kPropRows - 1 : p.ny - 1;
+    const int bx_last = bx0 + kPropCols - 1 < p.nx - 1 ? bx0 + kPropCols - 1 : p.nx - 1;
     const int ew2 = lat_count(p.width - 7);
-    const int B = 8;
-    const int G = p.group;
     const int K = a.topk;
-    const int M = K < kMaxProposals / 2 ? K : kMaxProposals / 2;
-    const int last_bi = bi0 + kPropCols - 1 < p.nx - 1 ? bi0 + kPropCols - 1 : p.nx - 1;
-    const bool shortcut = G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0);
-    int jr[kPropRows], tr[kPropRows], br[kPropRows], bjr[kPropRows];
-    bool rok[kPropRows];
-#pragma unroll
-    for (int m = 0; m < kPropRows; ++m) {
-        rok[m] = bj0 + m < p.ny;
-        bjr[m] = rok[m] ? bj0 + m : p.ny - 1;
-        jr[m] = grid_pos(bjr[m], p.block_step, p.height - B);
-        tr[m] = search_lo(jr[m], p.range, 1);
-        br[m] = search_hi(jr[m], p.height - B, p.range, 1);
+    const int M = K < kPropSlots ? K : kPropSlots;
+    // block m of the tile: row by0 + m / 2, column bx0 + m % 2 (clamped to the
+    // plane's blocks; the copies past the edge score nothing)
+    auto tile_block = [&](int m, int& by, int& bx) {
+        by = by0 + (m >> 1);
+        bx = bx0 + (m & 1);
+        const bool ok = by <= by_last && bx <= bx_last;
+        by = by <= by_last ? by : by_last;
+        bx = bx <= bx_last ? bx : bx_last;
+        return ok;
+    };
+    // union of the windows (lattice units from the first row/column)
+    const LatWin wf = lat_window(p, by0, bx0), wl = lat_window(p, by_last, bx_last);
+    const int uy0 = wf.y0, ux0 = wf.x0;
+    const int nyu = (wl.y0 + kLat * (wl.ny - 1) - uy0) / kLat + 1;
+    const int nxu = (wl.x0 + kLat * (wl.nx - 1) - ux0) / kLat + 1;
+    const int ncb = nxu > 0 ? (nxu + 15) >> 4 : 0;  // 1 or 2 (nxu <= 29)
+    const int ntile = nyu > 0 ? nyu * ncb : 0;
+    // A fragments: lane r <-> block r, dims 8q .. 8q+7 (the same k assignment
+    // as the B fragments)
+    h8 af;
+    {
+        int by, bx;
+        tile_block(r, by, bx);
+        __builtin_memcpy(&af, a.remb + ((size_t)by * p.nx + bx) * kEmb + 8 * q, sizeof(h8));
     }
-
-    if (!shortcut) {
-        // A fragments: lane r <-> block bi0 + r; D rows 4q + ii <-> block bi0 + 4q + ii
-        const int bia = bi0 + r <= last_bi ? bi0 + r : last_bi;
-        int lx0[4], nlx[4];
+    // D rows 4q + ii <-> blocks 4q + ii: their windows in union lattice units
+    int row_lo[4], nly[4], col_lo[4], nlx[4];
+    float nref[4];
 #pragma unroll
-        for (int ii = 0; ii < 4; ++ii) {
-            const int bi = bi0 + 4 * q + ii;
-            const int ic = grid_pos(bi <= last_bi ? bi : last_bi, p.block_step, p.width - B);
-            const int l = search_lo(ic, p.range, 1), rr = search_hi(ic, p.width - B, p.range, 1);
-            lx0[ii] = lat_first(l);
-            nlx[ii] = bi <= last_bi ? lat_span(lx0[ii], rr) : 0;
-        }
-        const int u_l = search_lo(grid_pos(bi0, p.block_step, p.width - B), p.range, 1);
-        const int u_r = search_hi(grid_pos(last_bi, p.block_step, p.width - B), p.width - B, p.range, 1);
-        const int ux0 = lat_first(u_l);
-        const int nux = lat_span(ux0, u_r);
-        const int ncb = (nux + 15) >> 4;
+    for (int ii = 0; ii < 4; ++ii) {
+        int by, bx;
+        const bool ok = tile_block(4 * q + ii, by, bx);
+        const LatWin w = lat_window(p, by, bx);
+        row_lo[ii] = (w.y0 - uy0) / kLat;
+        col_lo[ii] = (w.x0 - ux0) / kLat;
+        nly[ii] = ok ? w.ny : 0;
+        nlx[ii] = w.nx;
+        nref[ii] = a.rnorm[(size_t)by * p.nx + bx];
+    }
+    for (int e = threadIdx.x; e < 16 * kSubCap; e += 256) (&sub[0][0])[e] = 0xFFFFu;
+    __syncthreads();
+    // step t: union lattice row t / ncb, columns 16 (t % ncb) .. +15 (ncb is 1
+    // or 2: the union is at most 25 + 4 lattice columns wide); a wave scores
+    // steps wave, wave + 4, ..., i.e. a fixed column chunk and every
+    // (4 / ncb)-th row.  Loads run kPropDepth steps ahead, clamped to the last
+    // union row past the end (never used); the unrolled body has no exit, so
+    // the loads stay in flight across steps.
+    const int lrow0 = uy0 / kLat, lcol0 = ux0 / kLat;
+    const int dm = ncb > 0 ? 4 / ncb : 0;
+    const int mr0 = ncb > 0 ? wave / ncb : 0;
+    const int xi = (ncb > 0 ? wave % ncb : 0) * 16 + r;
+    const int nsteps = ntile > wave ? (ntile - wave + 3) / 4 : 0;
+    const size_t col_off = (size_t)lcol0 + (xi < nxu ? xi : nxu - 1);
+    auto load_step = [&](int k, h8& b, float& nq) {
+        const int mr = mr0 + dm * k;
+        const size_t pq = (size_t)(lrow0 + (mr < nyu ? mr : nyu - 1)) * ew2 + col_off;
+        __builtin_memcpy(&b, a.emb + pq * kEmb + 8 * q, sizeof(h8));
+        nq = a.norm[pq];
+    };
+    int rx[4];
+    bool colok[4];
 #pragma unroll
-        for (int m = 0; m < kPropRows; ++m) {
-            if (!rok[m]) break;
-            h4 af0, af1;
-            {
-                const _Float16* er = a.remb + ((size_t)bjr[m] * p.nx + bia) * kEmb;
-                __builtin_memcpy(&af0, er + 4 * q, sizeof(h4));
-                __builtin_memcpy(&af1, er + 16 + 4 * q, sizeof(h4));
-            }
-            float nref[4];
+    for (int ii = 0; ii < 4; ++ii) {
+        rx[ii] = xi - col_lo[ii];
+        colok[ii] = (unsigned)rx[ii] < (unsigned)nlx[ii];
+    }
+    h8 pb[kPropDepth];
+    float pnq[kPropDepth];
+    if (nsteps > 0) {
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) {
-                const int bi = bi0 + 4 * q + ii;
-                nref[ii] = a.rnorm[(size_t)bjr[m] * p.nx + (bi <= last_bi ? bi : last_bi)];
-            }
-            for (int e = threadIdx.x; e < kPropCols * kSubCap; e += 256) (&sub[0][0])[e] = 0xFFFFu;
-            __syncthreads();
-            const int ly0 = lat_first(tr[m]);
-            const int nly = lat_span(ly0, br[m]);
-            const int ntile = nly * ncb;
-            // one step = 16 lattice positions of one lattice row; the next step's
-            // embeddings are loaded while this one is scored
-            auto load_step = [&](int tl, h4& b0, h4& b1, float& nq) {
-                const int mr = tl / ncb;
-                const int xi = (tl - mr * ncb) * 16 + r;
-                const int x = ux0 + kLat * (xi < nux ? xi : nux - 1);
-                const size_t pq = (size_t)(ly0 / kLat + mr) * ew2 + x / kLat;
-                __builtin_memcpy(&b0, a.emb + pq * kEmb + 4 * q, sizeof(h4));
-                __builtin_memcpy(&b1, a.emb + pq * kEmb + 16 + 4 * q, sizeof(h4));
-                nq = a.norm[pq];
-            };
-            h4 n0, n1;
-            float nnq = 0.0f;
-            if (wave < ntile) load_step(wave, n0, n1, nnq);
-            for (int tl = wave; tl < ntile; tl += 4) {
-                const int mr = tl / ncb;
-                const int xi = (tl - mr * ncb) * 16 + r;
-                const int x = ux0 + kLat * (xi < nux ? xi : nux - 1);
-                const h4 b0 = n0, b1 = n1;
-                const float nq = nnq;
-                if (tl + 4 < ntile) load_step(tl + 4, n0, n1, nnq);
-                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc = mfma_h(af0, b0, acc);
-                acc = mfma_h(af1, b1, acc);
-                if (xi >= nux) continue;
+        for (int s = 0; s < kPropDepth; ++s) load_step(s, pb[s], pnq[s]);
+    }
+    for (int k0 = 0; k0 < nsteps; k0 += kPropDepth) {
+#pragma unroll
+        for (int s = 0; s < kPropDepth; ++s) {
+            const int k = k0 + s;
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            acc = mfma_h32(af, pb[s], acc);
+            const float nq = pnq[s];
+            load_step(k + kPropDepth, pb[s], pnq[s]);
+            if (k < nsteps) {
+                const int mr = mr0 + dm * k;
 #pragma unroll
                 for (int ii = 0; ii < 4; ++ii) {
-                    const int mc = (x - lx0[ii]) / kLat;
-                    if (x < lx0[ii] || mc >= nlx[ii]) continue;
+                    const int ry = mr - row_lo[ii];
+                    if (!colok[ii] || (unsigned)ry >= (unsigned)nly[ii]) continue;
                     float d = __builtin_fmaf(-2.0f, acc[ii], nref[ii] + nq);
                     d = d > 0.0f ? d : 0.0f;
-                    sub[4 * q + ii][mr * nlx[ii] + mc] = __builtin_bit_cast(unsigned short, (_Float16)d);
+                    sub[4 * q + ii][__umul24(ry, nlx[ii]) + rx[ii]] = __builtin_bit_cast(unsigned short, (_Float16)d);
                 }
             }
-            __syncthreads();
-            // per block: the M best lattice positions -> 3x3 neighbourhoods
-            for (int k = wave; k < kPropCols; k += 4) {
-                const int bi = bi0 + k;
-                if (bi > last_bi) break;
-                const int i = grid_pos(bi, p.block_step, p.width - B);
-                const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
-                const int ncols = rr - l + 1;
-                const int lxk = lat_first(l);
-                const int nlxk = lat_span(lxk, rr);
-                const int nsub = nly * nlxk;
-                const int j = jr[m], t = tr[m], bt = br[m];
-                const int self = (j - t) * ncols + (i - l);
-                constexpr int NS = (kSubCap + 63) / 64;
-                uint32_t v[NS];
-                uint32_t lmin = 0xFFFFu;
-#pragma unroll
-                for (int u = 0; u < NS; ++u) {
-                    const int e = lane + 64 * u;
-                    v[u] = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFu;
-                    lmin = v[u] < lmin ? v[u] : lmin;
-                }
-                uint32_t* bk = bits[wave];
-                for (int w = lane; w < kBitWords; w += 64) bk[w] = 0u;
-                // marks the 3x3 neighbourhood of lattice position e (self excluded)
-                auto mark = [&](int e) {
-                    const int my = e / nlxk;
-                    const int y0 = ly0 + kLat * my, x0 = lxk + kLat * (e - my * nlxk);
-#pragma unroll
-                    for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            const int yy = y0 + dy, xx = x0 + dx;
-                            if (yy < t || yy > bt || xx < l || xx > rr) continue;
-                            const int c = (yy - t) * ncols + (xx - l);
-                            if (c == self) continue;
-                            __hip_atomic_fetch_or(&bk[c >> 5], 1u << (c & 31), __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        }
-                };
-                // bound: the M-th smallest lane minimum tb (M lanes hold a value
-                // <= tb, so the M best by (score, position) all score <= tb);
-                // those values are compacted in position order, usually a few
-                // dozen, and the exact selection runs over them alone
-                uint32_t tb = 0;
-                for (int bit = 15; bit >= 0; --bit) {
-                    const uint32_t trial = tb | (1u << bit);
-                    if (__popcll(__ballot(lmin < trial)) < M) tb = trial;
-                }
-                uint32_t* ck = comp[wave];
-                int* ce = reinterpret_cast<int*>(ck + 64);
-                int ncomp = 0;
-#pragma unroll
-                for (int u = 0; u < NS; ++u) {
-                    const bool in = v[u] <= tb && v[u] != 0xFFFFu;
-                    const uint64_t mk = __ballot(in);
-                    const int at = ncomp + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
-                    if (in && at < 64) {
-                        ck[at] = v[u];
-                        ce[at] = lane + 64 * u;
-                    }
-                    ncomp += __popcll(mk);
-                }
-                wave_sync_lds();
-                if (ncomp <= 64) {
-                    const uint32_t cv = lane < ncomp ? ck[lane] : 0xFFFFu;
-                    uint32_t tv = 0;
-                    for (int bit = 15; bit >= 0; --bit) {
-                        const uint32_t trial = tv | (1u << bit);
-                        if (__popcll(__ballot(cv < trial)) < M) tv = trial;
-                    }
-                    const int less = __popcll(__ballot(cv < tv && cv != 0xFFFFu));
-                    const bool eq = cv == tv && cv != 0xFFFFu;
-                    const uint64_t em = __ballot(eq);
-                    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(em >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)em, 0));
-                    if ((cv < tv && cv != 0xFFFFu) || (eq && rank < M - less)) mark(ce[lane]);
-                } else {
-                    const uint32_t sel = wave_smallest16<NS>(v, M);
-#pragma unroll
-                    for (int u = 0; u < NS; ++u)
-                        if (sel & (1u << u)) mark(lane + 64 * u);
-                }
-                wave_sync_lds();
-                // the block's candidate bitmap goes to k_rerank
-                uint32_t* out = a.cand_bits + (size_t)(bjr[m] * p.nx + bi) * kBitWords;
-                for (int w = lane; w < kBitWords; w += 64) out[w] = bk[w];
-                wave_sync_lds();  // bk is cleared for this wave's next block
-            }
-            __syncthreads();  // sub is rewritten by the next tile row
         }
+    }
+    __syncthreads();
+    // per block: the M best lattice positions, as a list in lattice order
+    for (int k = wave; k < 16; k += 4) {
+        int by, bx;
+        if (!tile_block(k, by, bx)) continue;
+        const LatWin w = lat_window(p, by, bx);
+        const int nsub = w.ny * w.nx;
+        constexpr int NS = (kSubCap + 63) / 64;
+        uint32_t v[NS];
+        uint32_t lmin = 0xFFFFu;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = lane + 64 * u;
+            v[u] = e < nsub ? (uint32_t)sub[k][e] : 0xFFFFu;
+            lmin = v[u] < lmin ? v[u] : lmin;
+        }
+        uint16_t* out = a.props + ((size_t)by * p.nx + bx) * kPropSlots;
+        // bound: the M-th smallest lane minimum tb (M lanes hold a value
+        // <= tb, so the M best by (score, position) all score <= tb);
+        // those values are compacted in position order, usually a few
+        // dozen, and the exact selection runs over them alone
+        uint32_t tb = 0;
+        for (int bit = 15; bit >= 0; --bit) {
+            const uint32_t trial = tb | (1u << bit);
+            if (__popcll(__ballot(lmin < trial)) < M) tb = trial;
+        }
+        uint32_t* ck = comp[wave];
+        int* ce = reinterpret_cast<int*>(ck + 64);
+        int ncomp = 0;
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const bool in = v[u] <= tb && v[u] != 0xFFFFu;
+            const uint64_t mk = __ballot(in);
+            const int at = ncomp + mbcnt64(mk);
+            if (in && at < 64) {
+                ck[at] = v[u];
+                ce[at] = lane + 64 * u;
+            }
+            ncomp += __popcll(mk);
+        }
+        wave_sync_lds();
+        int nsel;
+        if (ncomp <= 64) {
+            const uint32_t cv = lane < ncomp ? ck[lane] : 0xFFFFu;
+            uint32_t tv = 0;
+            for (int bit = 15; bit >= 0; --bit) {
+                const uint32_t trial = tv | (1u << bit);
+                if (__popcll(__ballot(cv < trial)) < M) tv = trial;
+            }
+            const int less = __popcll(__ballot(cv < tv && cv != 0xFFFFu));
+            const bool eq = cv == tv && cv != 0xFFFFu;
+            const int rank = mbcnt64(__ballot(eq));
+            const bool sel = (cv < tv && cv != 0xFFFFu) || (eq && rank < M - less);
+            const uint64_t sm = __ballot(sel);
+            if (sel) out[mbcnt64(sm)] = (uint16_t)ce[lane];
+            nsel = __popcll(sm);
+        } else {
+            const uint32_t sel = wave_smallest16<NS>(v, M);
+            nsel = 0;
+#pragma unroll
+            for (int u = 0; u < NS; ++u) {
+                const bool s_ = (sel >> u) & 1u;
+                const uint64_t sm = __ballot(s_);
+                if (s_) out[nsel + mbcnt64(sm)] = (uint16_t)(lane + 64 * u);
+                nsel += __popcll(sm);
+            }
+        }
+        if (lane >= nsel && lane < kPropSlots) out[lane] = 0xFFFFu;
+        wave_sync_lds();  // comp is rewritten for this wave's next block
     }
 }
 
-// Per reference block (one wave): candidates from the bitmap of k_propose
-// (window scan order), exact re-rank, output.  A separate kernel so that the
-// latency-bound re-rank runs at the occupancy of its own small LDS footprint.
+// k_rerank: one workgroup (four waves) per run of kRerankBlocks consecutive
+// reference blocks of one block row, two blocks per wave (lanes 0-31 and
+// 32-63), one lane per proposal.  The union of the run's search windows
+// (with the pixels their candidate blocks cover, one padding row above and
+// below and two padding columns on the left; coordinates clamped to the
+// plane) is staged in LDS with coalesced loads: at most 58 x 117 floats at
+// bm_range 24.  Each lane reads the 10x10 pixels under its 3x3 neighbourhood
+// of candidates (rows cy-1 .. cy+8, columns cx-1 .. cx+8, as 8-byte aligned
+// pairs: cx is even) and evaluates the nine SSDs from registers against the
+// reference block, in the SSE2 lane order of ssd_fixed<8> (scalar: the
+// row-major order of builds without __SSE2__, Block.h:632-643): per candidate
+// rows in increasing order, each row's columns 0-3 and then 4-7 into the four
+// lane sums.  Selection per block: a radix descent over the leading bits of
+// all keys bounds the kk-th smallest (kk = min(accepted, G-1)); the keys at or
+// below the bound are compacted (into the window's LDS, free once every wave
+// has its SSDs) and ranked by (key, window index).
 constexpr int kRerankWaves = 4;
-constexpr int kRerankWords = 2 * kListCap + 4 * kMaxLearnedGroupSize + 64;
+constexpr int kRerankBlocks = 2 * kRerankWaves;
+constexpr int kWinRows = 49 + 7 + 2;  // search rows + block rows - 1 + padding above and below
+constexpr int kWinStride = 120;       // >= 49 + 7 * 8 + 13 columns, a multiple of four
+constexpr int kLatWords = (kSubCap + 31) / 32;  // lattice selection bitmap of one block
+constexpr int kCompCap = 9 * kPropSlots;        // every candidate of a block
+static_assert(kRerankBlocks * 2 * kCompCap <= kWinRows * kWinStride, "selection lists reuse the window");
+struct RerankLds {
+    alignas(16) float win[kWinRows * kWinStride];
+    float ref[kRerankBlocks][64];
+    uint32_t sel[kRerankBlocks][kLatWords];
+};
+
 template <bool SCALAR>
-__device__ __forceinline__ void rerank_body(const ProposeArgs& a, uint32_t (*smem)[kRerankWords]) {
+__device__ __forceinline__ void rerank_run(const ProposeArgs& a, RerankLds& s) {
     const StageParams& p = a.p;
     const int lane = lane_id();
+    const int h = lane >> 5, sl = lane & 31;
     const int wave = uniform((int)(threadIdx.x >> 6));
-    const int b = (int)blockIdx.x * kRerankWaves + wave;
-    if (b >= p.nx * p.ny) return;
     const int B = 8;
     const int G = p.group;
-    const int K = a.topk;
-    uint32_t* cand = smem[wave];
-    uint32_t* keys = cand + kListCap;
-    uint32_t* sel_key = keys + kListCap;
-    int* sel_idx = reinterpret_cast<int*>(sel_key + 2 * kMaxLearnedGroupSize);
-    float* rbs = reinterpret_cast<float*>(sel_idx + 2 * kMaxLearnedGroupSize);
-    const int by = b / p.nx, bi = b - by * p.nx;
+    // the run of the workgroup (XCD-contiguous, row-major)
+    const int runs_x = (p.nx + kRerankBlocks - 1) / kRerankBlocks;
+    const int nruns = runs_x * p.ny;
+    const int per_xcd = (nruns + 7) >> 3;
+    const int lin = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (lin >= nruns) return;
+    const int by = lin / runs_x;
+    const int bx0 = (lin - by * runs_x) * kRerankBlocks;
+    const int bxe = bx0 + kRerankBlocks - 1 < p.nx - 1 ? bx0 + kRerankBlocks - 1 : p.nx - 1;
+    const int bx = bx0 + 2 * wave + h;
+    const bool bok = bx <= bxe;
+    const int bxc = bok ? bx : bxe;
+    const int b = by * p.nx + bxc;
     const int j = grid_pos(by, p.block_step, p.height - B);
-    const int i = grid_pos(bi, p.block_step, p.width - B);
+    const int i = grid_pos(bxc, p.block_step, p.width - B);
     int* cnt_out = a.count + b;
     int* pos_out = a.pos + (size_t)b * G;
     float* key_out = a.key + (size_t)b * G;
     if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
-        if (lane == 0) {
+        if (sl == 0 && bok) {
             *cnt_out = 1;
             pos_out[0] = (j << 16) | i;
             key_out[0] = 0.0f;
@@ -849,90 +1010,222 @@ __device__ __forceinline__ void rerank_body(const ProposeArgs& a, uint32_t (*sme
     }
     const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
     const int l = search_lo(i, p.range, 1), rr = search_hi(i, p.width - B, p.range, 1);
-    const int nrows = bt - t + 1;
-    const int ncols = rr - l + 1, nc = ncols * nrows;
-    const int self = (j - t) * ncols + (i - l);
-    rbs[lane] = a.plane[(size_t)(j + (lane >> 3)) * p.pitch + i + (lane & 7)];
-    int n = 0;
-    if (nc - 1 <= K) {
-        for (int c0 = 0; c0 < nc; c0 += 64) {
-            const int c = c0 + lane;
-            const bool in = c < nc && c != self;
-            const uint64_t mk = __ballot(in);
-            const int at = n + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0));
-            if (in) cand[at] = (uint32_t)c;
-            n += __popcll(mk);
-        }
-    } else {
-        // the set bits of the block's bitmap, in window order
-        const int nw = (nc + 31) >> 5;
-        const uint32_t* bw = a.cand_bits + (size_t)b * kBitWords;
-        const uint32_t w0 = lane < nw ? bw[lane] : 0u;
-        const uint32_t w1 = lane + 64 < nw ? bw[lane + 64] : 0u;
-        const int c0n = __popc(w0), c1n = __popc(w1);
-        int incl = c0n, incl1 = c1n;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v0 = __shfl_up(incl, o), v1 = __shfl_up(incl1, o);
-            if (lane >= o) {
-                incl += v0;
-                incl1 += v1;
+    // the staged window: plane rows t-1 .. bt+8, columns L0 .. L0 + wc - 1
+    const int l_first = search_lo(grid_pos(bx0, p.block_step, p.width - B), p.range, 1);
+    const int rr_last = search_hi(grid_pos(bxe, p.block_step, p.width - B), p.width - B, p.range, 1);
+    const int L0 = (l_first - 2) & ~1;
+    const int wc = rr_last + 10 - L0;
+    const int wr = bt - t + 10;
+    const bool inner = t >= 1 && bt + 8 <= p.height - 1 && L0 >= 0 && L0 + ((wc + 1) & ~1) <= p.width &&
+                       (p.pitch & 1) == 0 && ((uintptr_t)a.plane & 7) == 0;
+    if (inner) {
+        // column pairs: thread -> pair tid % 60 of rows tid / 60, +4, +8, ...
+        constexpr int kPairs = kWinStride / 2;
+        constexpr int kRowsPer = 64 * kRerankWaves / kPairs;
+        const int c2 = (int)threadIdx.x % kPairs, r0 = (int)threadIdx.x / kPairs;
+        if (r0 < kRowsPer && 2 * c2 < wc) {
+            const float* src = a.plane + (size_t)(t - 1 + r0) * p.pitch + L0 + 2 * c2;
+            float* dst = s.win + r0 * kWinStride + 2 * c2;
+#pragma unroll 4
+            for (int r = r0; r < wr; r += kRowsPer) {
+                float2 v;
+                __builtin_memcpy(&v, src, sizeof(v));
+                __builtin_memcpy(dst, &v, sizeof(v));
+                src += (size_t)kRowsPer * p.pitch;
+                dst += kRowsPer * kWinStride;
             }
         }
-        const int tot0 = __shfl(incl, 63);
-        n = tot0 + __shfl(incl1, 63);
-        int at = incl - c0n;
-        for (uint32_t w = w0; w; w &= w - 1) cand[at++] = (uint32_t)(lane * 32 + __builtin_ctz(w));
-        at = tot0 + incl1 - c1n;
-        for (uint32_t w = w1; w; w &= w - 1) cand[at++] = (uint32_t)((lane + 64) * 32 + __builtin_ctz(w));
-    }
-    wave_sync_lds();
-    // exact SSD of every candidate, one lane per candidate (reference block
-    // read from LDS by broadcast)
-    int nacc = 0;
-    for (int e0 = 0; e0 < n; e0 += 64) {
-        const int e = e0 + lane;
-        uint32_t kv = 0xFFFFFFFFu;
-        if (e < n) {
-            const int c = (int)cand[e];
-            const int cy = c / ncols, cx = c - cy * ncols;
-            const float dist = ssd8_rows<SCALAR>(rbs, a.plane + (size_t)(t + cy) * p.pitch + l + cx, p.pitch);
-            if (dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
-            keys[e] = kv;
+    } else {
+        // windows at the plane's edges: every sample with clamped coordinates
+        for (int e = (int)threadIdx.x; e < wr * kWinStride; e += 64 * kRerankWaves) {
+            const int r = e / kWinStride, c = e - r * kWinStride;
+            if (c >= wc) continue;
+            int y = t - 1 + r, x = L0 + c;
+            y = y < 0 ? 0 : (y > p.height - 1 ? p.height - 1 : y);
+            x = x < 0 ? 0 : (x > p.width - 1 ? p.width - 1 : x);
+            s.win[e] = a.plane[(size_t)y * p.pitch + x];
         }
-        nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
     }
+    const int slot = 2 * wave + h;
+    float* rf = s.ref[slot];
+    uint32_t* sb = s.sel[slot];
+    rf[sl] = a.plane[(size_t)(j + (sl >> 3)) * p.pitch + i + (sl & 7)];
+    rf[sl + 32] = a.plane[(size_t)(j + 4 + (sl >> 3)) * p.pitch + i + (sl & 7)];
+    for (int w = sl; w < kLatWords; w += 32) sb[w] = 0u;
+    const int ncols = rr - l + 1;
+    const int ly0 = lat_first(t), lx0 = lat_first(l);
+    const int nly = lat_span(ly0, bt), nlx = lat_span(lx0, rr);
+    const int M = a.topk < kPropSlots ? a.topk : kPropSlots;
+    const bool full = nly * nlx <= M;  // every lattice position proposed
+    const int e = bok ? (int)a.props[(size_t)b * kPropSlots + sl] : 0xFFFF;
+    const bool act = e < nly * nlx;  // 0xFFFF: no proposal in this slot
+    const int my = act ? e / nlx : 0;
+    const int mx = act ? e - my * nlx : 0;
+    __syncthreads();
+    if (act) __hip_atomic_fetch_or(&sb[e >> 5], 1u << (e & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     wave_sync_lds();
-    if (nacc < G - 1 && nc - 1 > K) {
-        // not enough acceptable candidates: the exhaustive search decides
-        if (lane == 0) {
-            const int at = atomicAdd(a.nfallback, 1);
-            a.fallback[at] = b;
+    auto proposed = [&](int yy, int xx) -> bool {
+        if (yy < 0 || xx < 0 || xx >= nlx) return false;
+        const int idx = yy * nlx + xx;
+        return (sb[idx >> 5] >> (idx & 31)) & 1u;
+    };
+    // candidates this lane owns: inside the window, not self, and not in the
+    // neighbourhood of an earlier proposal (rows above, or the left neighbour)
+    const bool up = proposed(my - 1, mx), upl = proposed(my - 1, mx - 1), upr = proposed(my - 1, mx + 1);
+    const bool lf = proposed(my, mx - 1);
+    const int cy = ly0 + kLat * my, cx = lx0 + kLat * mx;
+    uint32_t own = 0;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int y = cy + dy, x = cx + dx;
+            bool ok = act && y >= t && y <= bt && x >= l && x <= rr && !(y == j && x == i);
+            if (dy == -1 && (up || (dx == -1 && upl) || (dx == 1 && upr))) ok = false;
+            if (dx == -1 && lf) ok = false;
+            own |= (ok ? 1u : 0u) << ((dy + 1) * 3 + dx + 1);
         }
-        return;
+    // the nine SSDs; patch row pr = plane row cy - 1 + pr = window row cy - t + pr
+    const float* wbase = s.win + (cy - t) * kWinStride + (cx - 2 - L0);
+    float acc[9][SCALAR ? 1 : 4];
+#pragma unroll
+    for (int u = 0; u < 9; ++u)
+#pragma unroll
+        for (int c = 0; c < (SCALAR ? 1 : 4); ++c) acc[u][c] = 0.0f;
+    auto load_row = [&](int pr, float (&v)[10]) {
+        float2 w[6];
+        const float2* src = reinterpret_cast<const float2*>(wbase + pr * kWinStride);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) w[k] = src[k];
+        v[0] = w[0].y;
+        v[1] = w[1].x;
+        v[2] = w[1].y;
+        v[3] = w[2].x;
+        v[4] = w[2].y;
+        v[5] = w[3].x;
+        v[6] = w[3].y;
+        v[7] = w[4].x;
+        v[8] = w[4].y;
+        v[9] = w[5].x;
+    };
+    // patch row pr is row pr - dy of the candidates in neighbourhood row dy
+    auto add_row = [&](int pr, const float (&v)[10]) {
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+            const int rrow = pr - dy;
+            if (rrow < 0 || rrow > 7) continue;
+            float rb[8];
+            __builtin_memcpy(&rb[0], rf + rrow * 8, 8 * sizeof(float));
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                float* sacc = acc[dy * 3 + dx];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const float d = rb[c] - v[dx + c];
+                    if constexpr (SCALAR) sacc[0] = sacc[0] + d * d;
+                    else sacc[c & 3] = sacc[c & 3] + d * d;
+                }
+            }
+        }
+    };
+    float va[10], vb[10];
+    load_row(0, va);
+#pragma unroll 1
+    for (int pr = 0; pr < 10; pr += 2) {
+        load_row(pr + 1, vb);
+        add_row(pr, va);
+        if (pr + 2 < 10) load_row(pr + 2, va);
+        add_row(pr + 1, vb);
+    }
+    // keys (0xFFFFFFFF: not a candidate or not accepted); the half-wave counts
+    // are taken from the two halves of each ballot
+    uint32_t kv[9];
+    int nacc_lo = 0, nacc_hi = 0;
+#pragma unroll
+    for (int u = 0; u < 9; ++u) {
+        float dist;
+        if constexpr (SCALAR) dist = acc[u][0];
+        else dist = 0.0f + (((acc[u][0] + acc[u][1]) + acc[u][2]) + acc[u][3]);
+        const bool ok = ((own >> u) & 1u) && dist <= p.th_sse && dist != 0.0f;
+        kv[u] = ok ? __float_as_uint((float)((double)dist * p.dist_mul)) : 0xFFFFFFFFu;
+        const uint64_t bm = __ballot(ok);
+        nacc_lo += __popc((uint32_t)bm);
+        nacc_hi += __popc((uint32_t)(bm >> 32));
+    }
+    const int nacc = h ? nacc_hi : nacc_lo;
+    const bool fb = nacc < G - 1 && !full;  // the exhaustive search decides
+    if (fb && sl == 0 && bok) {
+        const int at = atomicAdd(a.nfallback, 1);
+        a.fallback[at] = b;
     }
     const int kk = nacc < G - 1 ? nacc : G - 1;
-    // the kk best by (key, list position); list position order = window order
-    wave_select_small<(kListCap + 63) / 64>(keys, n, nacc, kk, sel_key, sel_idx);
-    for (int z = lane; z < kk; z += 64) {
-        const int c = (int)cand[sel_idx[kk + z]];
-        const int cy = c / ncols;
-        pos_out[1 + z] = ((t + cy) << 16) | (l + c - cy * ncols);
-        key_out[1 + z] = __uint_as_float(sel_key[kk + z]);
+    // tb: an upper bound of the kk-th smallest key from its 16 leading bits
+    // (keys are positive floats: bit 31 is clear), so that the compacted set
+    // holds the kk best plus the few keys sharing their leading bits
+    uint32_t tlo = 0, thi = 0;
+    const int kk_lo = nacc_lo < G - 1 ? nacc_lo : G - 1, kk_hi = nacc_hi < G - 1 ? nacc_hi : G - 1;
+    for (int bit = 30; bit >= 15; --bit) {
+        const uint32_t trl = tlo | (1u << bit), trh = thi | (1u << bit);
+        const uint32_t trial = h ? trh : trl;
+        int clo = 0, chi = 0;
+#pragma unroll
+        for (int u = 0; u < 9; ++u) {
+            const uint64_t bm = __ballot(kv[u] < trial);
+            clo += __popc((uint32_t)bm);
+            chi += __popc((uint32_t)(bm >> 32));
+        }
+        if (clo < kk_lo) tlo = trl;
+        if (chi < kk_hi) thi = trh;
     }
-    if (lane == 0) {
-        *cnt_out = kk + 1;
-        pos_out[0] = (j << 16) | i;
-        key_out[0] = 0.0f;
+    const uint32_t tb = (h ? thi : tlo) | 0x7FFFu;
+    __syncthreads();  // every wave is done with the window
+    uint32_t* ck = reinterpret_cast<uint32_t*>(s.win) + slot * 2 * kCompCap;
+    int* ci = reinterpret_cast<int*>(ck + kCompCap);
+    int nlo = 0, nhi = 0;
+#pragma unroll
+    for (int u = 0; u < 9; ++u) {
+        const bool in = kv[u] <= tb && kv[u] != 0xFFFFFFFFu;
+        const uint64_t m = __ballot(in);
+        if (in) {
+            const int dy = u / 3 - 1, dx = u % 3 - 1;
+            const int at = (h ? nhi : nlo) + mbcnt64(m) - (h ? __popc((uint32_t)m) : 0);
+            ck[at] = kv[u];
+            ci[at] = (cy + dy - t) * ncols + (cx + dx - l);
+        }
+        nlo += __popc((uint32_t)m);
+        nhi += __popc((uint32_t)(m >> 32));
+    }
+    const int ncomp = h ? nhi : nlo;
+    wave_sync_lds();
+    if (!fb && bok) {
+        for (int e1 = sl; e1 < ncomp; e1 += 32) {
+            const uint32_t k1 = ck[e1];
+            const int w1 = ci[e1];
+            int rank = 0;
+            for (int e2 = 0; e2 < ncomp; ++e2) {
+                const uint32_t k2 = ck[e2];
+                rank += (k2 < k1) || (k2 == k1 && ci[e2] < w1);
+            }
+            if (rank < kk) {
+                const int wy = w1 / ncols;
+                pos_out[1 + rank] = ((t + wy) << 16) | (l + w1 - wy * ncols);
+                key_out[1 + rank] = __uint_as_float(k1);
+            }
+        }
+        if (sl == 0) {
+            *cnt_out = kk + 1;
+            pos_out[0] = (j << 16) | i;
+            key_out[0] = 0.0f;
+        }
     }
 }
 
-// One wave per reference block; the arithmetic flavour is chosen once per
-// launch (the LDS lives here so that both bodies share it).
+// The arithmetic flavour is chosen once per launch (the LDS lives here so
+// that both bodies share it).
 __global__ __launch_bounds__(64 * kRerankWaves) void k_rerank(ProposeArgs a) {
-    __shared__ uint32_t smem[kRerankWaves][kRerankWords];
-    if (a.p.arith) rerank_body<true>(a, smem);
-    else rerank_body<false>(a, smem);
+    __shared__ RerankLds smem;
+    if (a.p.arith) rerank_run<true>(a, smem);
+    else rerank_run<false>(a, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -983,8 +1276,7 @@ hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int 
                         hipStream_t st) {
     const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
     if (ew2 <= 0 || eh2 <= 0) return hipSuccess;
-    const int per_row = (ew2 + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
-    const int waves = per_row * eh2;
+    const int waves = ((ew2 + 15) / 16) * ((eh2 + kEmbedRun - 1) / kEmbedRun);
     hipLaunchKernelGGL(k_embed_lattice, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, pitch, W, H, emb, norm);
     return hipGetLastError();
 }
@@ -1002,12 +1294,12 @@ hipError_t launch_propose(const ProposeArgs& a, hipStream_t st) {
     const int tiles = ((a.p.ny + kPropRows - 1) / kPropRows) * ((a.p.nx + kPropCols - 1) / kPropCols);
     if (tiles == 0) return hipSuccess;
     hipLaunchKernelGGL(k_propose, dim3(8 * ((tiles + 7) / 8)), dim3(256), 0, st, a);
-    const int nb = a.p.nx * a.p.ny;
-    hipLaunchKernelGGL(k_rerank, dim3((nb + kRerankWaves - 1) / kRerankWaves), dim3(64 * kRerankWaves), 0, st, a);
+    const int runs = ((a.p.nx + kRerankBlocks - 1) / kRerankBlocks) * a.p.ny;
+    hipLaunchKernelGGL(k_rerank, dim3(8 * ((runs + 7) / 8)), dim3(64 * kRerankWaves), 0, st, a);
     return hipGetLastError();
 }
 
-size_t propose_bits_words(const StageParams& p) { return (size_t)p.nx * p.ny * kBitWords; }
+size_t propose_list_bytes(const StageParams& p) { return (size_t)p.nx * p.ny * kPropSlots * sizeof(uint16_t); }
 
 }  // namespace lbm
 }  // namespace bm3d
