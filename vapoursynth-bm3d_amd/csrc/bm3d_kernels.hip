@@ -504,6 +504,203 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
+// Sparse re-run over blist[0 .. *nlist) with the four waves of a workgroup
+// on one reference block at a time (the blocks are few, so the latency of one
+// block, not the throughput, sets the time): the workgroup stages the window,
+// wave w scores strip tasks w, w + 4, ... (task = lane + 64 (w + 4 tk)) with
+// match8_block's arithmetic, and each wave compacts its candidates at or below
+// a bound of its kk-th smallest key (kk = min(accepted, G - 1) over the whole
+// block), which holds every one of the block's kk best that the wave scored.
+// The workgroup then ranks the compacted pairs by (key, candidate index).
+// LDS: max(window + reference block, 4 waves x 64 KC W pairs) + 4 counts.
+constexpr int kCoopW = 10;  // strip width of the cooperative re-run
+__host__ __device__ constexpr size_t coop_words(int side, int KC, int W) {
+    const size_t win = (size_t)(side + 7) * (size_t)((side + 7) | 1) + (size_t)W - 1 + 64;
+    const size_t sel = (size_t)4 * 64 * KC * W * 2;
+    return (win > sel ? win : sel) + 4;
+}
+
+template <int KC, int W>
+__global__ __launch_bounds__(256) void k_match8_coop(const float* __restrict__ plane, StageParams p, int nb,
+                                                     const int* __restrict__ blist, const int* __restrict__ nlist,
+                                                     int* __restrict__ out_count, int* __restrict__ out_pos,
+                                                     float* __restrict__ out_key, int words) {
+    extern __shared__ uint32_t smem[];
+    constexpr int B = 8;
+    constexpr int NK = KC * W;
+    const int lane = lane_id();
+    const int wave = uniform((int)(threadIdx.x >> 6));
+    const int tid = (int)threadIdx.x;
+    const int G = p.group;
+    const int n = uniform(*nlist);
+    int* wcnt = reinterpret_cast<int*>(smem + words - 4);  // per wave: accepted, then compacted
+    float* win = reinterpret_cast<float*>(smem);
+    for (int k = blockIdx.x; k < n && k < nb; k += gridDim.x) {
+        const int b = uniform(blist[k]);
+        const int bj = b / p.nx, bi = b - bj * p.nx;
+        const int j = grid_pos(bj, p.block_step, p.height - B);
+        const int i = grid_pos(bi, p.block_step, p.width - B);
+        int* cnt_out = out_count + b;
+        int* pos_out = out_pos + (size_t)b * G;
+        float* key_out = out_key + (size_t)b * G;
+        if (G == 1 || !(p.th_sse > 0.0f && p.dist_mul > 0.0)) {
+            if (tid == 0) {
+                *cnt_out = 1;
+                pos_out[0] = (j << 16) | i;
+                key_out[0] = 0.0f;
+            }
+            continue;
+        }
+        const int l = search_lo(i, p.range, 1), r = search_hi(i, p.width - B, p.range, 1);
+        const int t = search_lo(j, p.range, 1), bt = search_hi(j, p.height - B, p.range, 1);
+        const int ncols = r - l + 1, nrows = bt - t + 1;
+        const int nstr = (ncols + W - 1) / W;
+        const int WW = (ncols + 7) | 1;
+        const int WH = nrows + 7;
+        const int nel = WH * WW + W - 1;
+        // the window, eight loads in flight per thread
+        for (int e0 = 0; e0 < nel; e0 += 8 * 256) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u + tid;
+                const int wy = e / WW, wx = e - wy * WW;
+                const int gx = l + wx;
+                v[u] = (e < nel && wy < WH && gx < p.width) ? plane[(size_t)(t + wy) * p.pitch + gx] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + 256 * u + tid;
+                if (e < nel) win[e] = v[u];
+            }
+        }
+        float* rbs = win + nel;  // [64] reference block
+        const int oy = j - t, ox = i - l;
+        __syncthreads();
+        if (wave == 0) rbs[lane] = win[(oy + (lane >> 3)) * WW + ox + (lane & 7)];
+        __syncthreads();
+        const int tasks = nrows * nstr;
+        uint32_t keys[NK];
+        int cbase[KC];
+        int nacc = 0;
+#pragma unroll
+        for (int tk = 0; tk < KC; ++tk) {
+            const int task = lane + 64 * (wave + 4 * tk);
+            const bool active = task < tasks;
+            const int strip = active ? task / nrows : 0;
+            const int cy = active ? task - strip * nrows : 0;
+            const int sx = strip * W;
+            cbase[tk] = cy * ncols + sx;
+            float s[W][4];
+#pragma unroll
+            for (int kx = 0; kx < W; ++kx)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) s[kx][c] = 0.0f;
+#pragma unroll 1
+            for (int rr = 0; rr < 8; ++rr) {
+                const float4 ra = *reinterpret_cast<const float4*>(rbs + rr * 8);
+                const float4 rc = *reinterpret_cast<const float4*>(rbs + rr * 8 + 4);
+                const float rv[8] = {ra.x, ra.y, ra.z, ra.w, rc.x, rc.y, rc.z, rc.w};
+                const float* wr = win + (cy + rr) * WW + sx;
+                float row[W + 7];
+#pragma unroll
+                for (int q = 0; q < W + 7; ++q) row[q] = wr[q];
+#pragma unroll
+                for (int kx = 0; kx < W; ++kx) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        float d = rv[c] - row[kx + c];
+                        s[kx][c] = s[kx][c] + d * d;
+                    }
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        float d = rv[4 + c] - row[kx + 4 + c];
+                        s[kx][c] = s[kx][c] + d * d;
+                    }
+                }
+            }
+#pragma unroll
+            for (int kx = 0; kx < W; ++kx) {
+                const float dist = 0.0f + (((s[kx][0] + s[kx][1]) + s[kx][2]) + s[kx][3]);
+                const int cx = sx + kx;
+                const bool valid = active && cx < ncols && !(cy == oy && cx == ox);
+                uint32_t kv = 0xFFFFFFFFu;
+                if (valid && dist <= p.th_sse && dist != 0.0f) kv = __float_as_uint((float)((double)dist * p.dist_mul));
+                keys[tk * W + kx] = kv;
+                nacc += __popcll(__ballot(kv != 0xFFFFFFFFu));
+            }
+        }
+        if (lane == 0) wcnt[wave] = nacc;
+        __syncthreads();  // every wave is done with the window; the counts are in
+        const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        const int kk = total < G - 1 ? total : G - 1;
+        // the wave's bound: its kk-th smallest key from the 16 leading bits
+        uint32_t tp = 0;
+        for (int bit = 30; bit >= 15; --bit) {
+            const uint32_t trial = tp | (1u << bit);
+            int cnt = 0;
+#pragma unroll
+            for (int q = 0; q < NK; ++q) cnt += __popcll(__ballot(keys[q] < trial));
+            if (cnt < kk) tp = trial;
+        }
+        const uint32_t tb = tp | 0x7FFFu;
+        uint32_t* ck = smem + (size_t)wave * (64 * NK * 2);
+        int* ci = reinterpret_cast<int*>(ck + 64 * NK);
+        int m = 0;
+#pragma unroll
+        for (int q = 0; q < NK; ++q) {
+            const bool in = keys[q] <= tb && keys[q] != 0xFFFFFFFFu;
+            const uint64_t bm = __ballot(in);
+            if (in) {
+                const int at = m + __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0));
+                ck[at] = keys[q];
+                ci[at] = cbase[q / W] + q % W;
+            }
+            m += __popcll(bm);
+        }
+        __syncthreads();  // the accepted counts have been read
+        if (lane == 0) wcnt[wave] = m;
+        __syncthreads();
+        const int m0 = wcnt[0], m1 = wcnt[1], m2 = wcnt[2], m3 = wcnt[3];
+        const int mt = m0 + m1 + m2 + m3;
+        auto entry = [&](int e, uint32_t& kv, int& iv) {
+            int w = 0;
+            if (e >= m0) { e -= m0; w = 1;
+                if (e >= m1) { e -= m1; w = 2;
+                    if (e >= m2) { e -= m2; w = 3; } } }
+            const uint32_t* kb = smem + (size_t)w * (64 * NK * 2);
+            kv = kb[e];
+            iv = reinterpret_cast<const int*>(kb + 64 * NK)[e];
+        };
+        for (int e1 = tid; e1 < mt; e1 += 256) {
+            uint32_t k1;
+            int i1;
+            entry(e1, k1, i1);
+            int rank = 0;
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t* kb = smem + (size_t)w * (64 * NK * 2);
+                const int* ib = reinterpret_cast<const int*>(kb + 64 * NK);
+                const int mw = w == 0 ? m0 : w == 1 ? m1 : w == 2 ? m2 : m3;
+                for (int e2 = 0; e2 < mw; ++e2) {
+                    const uint32_t k2 = kb[e2];
+                    rank += (k2 < k1) || (k2 == k1 && ib[e2] < i1);
+                }
+            }
+            if (rank < kk) {
+                const int cy = i1 / ncols;
+                pos_out[1 + rank] = ((t + cy) << 16) | (l + i1 - cy * ncols);
+                key_out[1 + rank] = __uint_as_float(k1);
+            }
+        }
+        if (tid == 0) {
+            *cnt_out = kk + 1;
+            pos_out[0] = (j << 16) | i;
+            key_out[0] = 0.0f;
+        }
+        __syncthreads();  // LDS is rewritten for the next block
+    }
+}
+
 // Frame of global item idx: the last f with starts[f] <= idx (starts[0] = 0,
 // ascending, nframes entries; wave-uniform).
 __device__ __forceinline__ int batch_frame(const int* __restrict__ starts, int nframes, int idx) {
@@ -1538,6 +1735,20 @@ hipError_t launch_match(const float* plane, const StageParams& p, int nb, int ma
     if (nb <= 0) return hipSuccess;
     // the 8x8 register-strip path computes the x86 (SSE2) lane order only;
     // the scalar flavour runs the generic matcher
+    if (blist && p.block == 8 && p.step == 1 && p.arith == kArithSSE2) {
+        // a block list: the cooperative form (four waves per block), windows up to 49 x 49
+        const int side = (int)(0.5 + __builtin_sqrt((double)max_cand));
+        const int nstr = (side + kCoopW - 1) / kCoopW;
+        if (side * nstr <= 256) {
+            const size_t words = coop_words(side, 1, kCoopW);
+            // enough workgroups that a typical list (a few percent of the blocks) takes
+            // one block per workgroup
+            const int grid = nb < device_cus() * 8 ? nb : device_cus() * 8;
+            hipLaunchKernelGGL((k_match8_coop<1, kCoopW>), dim3(grid), dim3(256), words * 4, st, plane, p, nb, blist,
+                               nlist, count, pos, key, (int)words);
+            return hipGetLastError();
+        }
+    }
     if (p.block == 8 && p.step == 1 && p.arith == kArithSSE2) {
         const Match8Shape sh = match8_shape(max_cand, p.group);
         const int km = sh.kmax;

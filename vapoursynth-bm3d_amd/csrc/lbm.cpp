@@ -41,10 +41,6 @@ namespace {
 constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
 constexpr float kLossT0 = 1.0f;
 
-__global__ void k_clear_int(int* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
-}
-
 int ensure_batch(bm3d_lbm* m, int npairs) {
     const size_t np = 2 * (size_t)npairs;
     int rc;
@@ -123,18 +119,18 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     const int W = sp.width, H = sp.height;
     const size_t npos = (size_t)((W - 7 + L::kLat - 1) / L::kLat) * ((H - 7 + L::kLat - 1) / L::kLat);  // lattice
     int rc;
-    if ((rc = w.emb.ensure(npos * L::kEmb * sizeof(_Float16)))) return rc;
-    if ((rc = w.norm.ensure(npos * sizeof(float)))) return rc;
-    if ((rc = w.remb.ensure((size_t)nb * L::kEmb * sizeof(_Float16)))) return rc;
-    if ((rc = w.rnorm.ensure((size_t)nb * sizeof(float)))) return rc;
+    // (one spare slot each: the embedding kernels' stores of patches without an output)
+    if ((rc = w.emb.ensure((npos + 1) * L::kEmb * sizeof(_Float16)))) return rc;
+    if ((rc = w.norm.ensure((npos + 1) * sizeof(float)))) return rc;
+    if ((rc = w.remb.ensure(((size_t)nb + 1) * L::kEmb * sizeof(_Float16)))) return rc;
+    if ((rc = w.rnorm.ensure(((size_t)nb + 1) * sizeof(float)))) return rc;
     if ((rc = w.fallback.ensure((size_t)nb * sizeof(int)))) return rc;
     if ((rc = w.nfallback.ensure(sizeof(int)))) return rc;
     if ((rc = w.props.ensure(L::propose_list_bytes(sp)))) return rc;
     const L::Net n = m->net();
     HIP_TRY(L::launch_embed(n, plane, sp.pitch, W, H, w.emb.as<_Float16>(), w.norm.as<float>(), st));
-    HIP_TRY(L::launch_embed_refs(n, plane, sp, w.remb.as<_Float16>(), w.rnorm.as<float>(), st));
-    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(64), 0, st, w.nfallback.as<int>());
-    HIP_TRY(hipGetLastError());
+    HIP_TRY(L::launch_embed_refs(n, plane, sp, w.remb.as<_Float16>(), w.rnorm.as<float>(), w.nfallback.as<int>(),
+                                 st));
     L::ProposeArgs pa{};
     pa.p = sp;
     pa.plane = plane;

@@ -54,10 +54,10 @@ struct TrainBatch {
 struct ProposeArgs {
     StageParams p;
     const float* plane;
-    const _Float16* emb;   // even lattice: [ceil((H-7)/2) * ceil((W-7)/2)][kEmb]
-    const float* norm;     // [ceil((H-7)/2) * ceil((W-7)/2)]
-    const _Float16* remb;  // reference blocks: [nx * ny][kEmb]
-    const float* rnorm;    // [nx * ny]
+    const _Float16* emb;   // even lattice: [ceil((H-7)/2) * ceil((W-7)/2) + 1][kEmb] (last: spare slot)
+    const float* norm;     // [ceil((H-7)/2) * ceil((W-7)/2) + 1]
+    const _Float16* remb;  // reference blocks: [nx * ny + 1][kEmb]
+    const float* rnorm;    // [nx * ny + 1]
     int max_cand;         // (2 range + 1)^2
     int topk;
     int* count;
@@ -77,10 +77,13 @@ hipError_t launch_train_adam(const Net& n, const float* grad, float* m, float* v
 hipError_t launch_sync_bf16(const Net& n, hipStream_t st);
 hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int npairs, float* target,
                            hipStream_t st);
+// emb/norm hold ceil((W-7)/2) * ceil((H-7)/2) + 1 entries (the last: spare slot), likewise nx * ny + 1 for
+// launch_embed_refs
 hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
                         hipStream_t st);
+// (also clears *nfallback for the re-rank that follows)
 hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
-                             hipStream_t st);
+                             int* nfallback, hipStream_t st);
 constexpr int kMaxLearnedGroupSize = 64;
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
 size_t propose_list_bytes(const StageParams& p);

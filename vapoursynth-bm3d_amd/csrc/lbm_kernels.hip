@@ -369,51 +369,11 @@ __device__ __forceinline__ sh2 relu_bf16x2(float a, float b) {
     return __builtin_elementwise_max(__builtin_bit_cast(sh2, __builtin_convertvector(v, bf2)), z);
 }
 
-// The network on 16 patches (inference form of forward16: same roundings).
-__device__ __forceinline__ void infer16(const Frags& f, const s4 (&x)[4], float (&e)[2][4]) {
-    s4 hb[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        f4 acc = {f.b1[t][0], f.b1[t][1], f.b1[t][2], f.b1[t][3]};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) acc = mfma(f.w1[t][s], x[s], acc);
-        const sh2 lo = relu_bf16x2(acc[0], acc[1]), hi = relu_bf16x2(acc[2], acc[3]);
-        hb[t] = s4{lo[0], lo[1], hi[0], hi[1]};
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-        f4 acc = {f.b2[u][0], f.b2[u][1], f.b2[u][2], f.b2[u][3]};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) acc = mfma(f.w2[u][s], hb[s], acc);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) e[u][i] = acc[i];
-    }
-}
-
-// The lane's part of the patch at (py, px): rows 2 s + q / 2, columns
-// 4 (q % 2) .. +3 for s = 0..3 (q = lane / 16), as loaded.
-__device__ __forceinline__ void patch_load(const float* __restrict__ plane, int pitch, int py, int px,
-                                           float4 (&v)[4]) {
-    const int q = lane_id() >> 4;
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-        __builtin_memcpy(&v[s], plane + (size_t)(py + 2 * s + (q >> 1)) * pitch + px + 4 * (q & 1), sizeof(float4));
-}
-
-// The MFMA fragments of a loaded patch part (packed bf16 conversion).
-__device__ __forceinline__ void patch_cvt(const float4 (&v)[4], s4 (&x)[4]) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const fl2 a = {v[s].x, v[s].y}, b = {v[s].z, v[s].w};
-        const sh2 lo = __builtin_bit_cast(sh2, __builtin_convertvector(a, bf2));
-        const sh2 hi = __builtin_bit_cast(sh2, __builtin_convertvector(b, bf2));
-        x[s] = s4{lo[0], lo[1], hi[0], hi[1]};
-    }
-}
-
 // Stores the embedding (fp16) and its squared norm (fp32 of the fp16 values)
-// of the lane's patch at index idx (idx < 0: no patch).
-__device__ __forceinline__ void store_emb(const float (&e)[2][4], int idx, _Float16* __restrict__ emb,
+// of the lane's patch at index idx.  Patches without an output (past the end
+// of a run or a list) store to the spare slot at index `spare`, so the stores
+// are unconditional and a run of patches stays one straight-line block.
+__device__ __forceinline__ void store_emb(const float (&e)[2][4], int idx, int spare, _Float16* __restrict__ emb,
                                           float* __restrict__ norm) {
     const int q = lane_id() >> 4;
     float nrm = 0.0f;
@@ -426,48 +386,15 @@ __device__ __forceinline__ void store_emb(const float (&e)[2][4], int idx, _Floa
             const float ev = (float)eh[u][i];
             nrm += ev * ev;
         }
-    // lanes r, r+16, r+32, r+48 hold the four quarters of one embedding
+    // lanes r, r+16, r+32, r+48 hold the four quarters of one embedding; the
+    // butterfly leaves the same sum in all four (fp addition commutes), and
+    // all four store it
     nrm += __shfl_xor(nrm, 16);
     nrm += __shfl_xor(nrm, 32);
-    if (idx >= 0) {
+    const size_t at = (size_t)(idx >= 0 ? idx : spare);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) __builtin_memcpy(emb + (size_t)idx * kEmb + 16 * u + 4 * q, &eh[u], sizeof(h4));
-        if (q == 0) norm[idx] = nrm;
-    }
-}
-
-// Each wave embeds kEmbedGroups groups of 16 patches; pos(g, r, y, x)
-// returns the output index of lane r's patch in group g (and its top-left
-// corner), or -1.  The patches of the next two groups are in flight while a
-// group runs (loaded as floats, converted when their group starts).
-constexpr int kEmbedGroups = 16;
-template <typename Pos>
-__device__ __forceinline__ void embed_groups(const Net& n, const float* __restrict__ plane, int pitch, Pos pos,
-                                             _Float16* __restrict__ emb, float* __restrict__ norm) {
-    const int r = lane_id() & 15;
-    Frags f;
-    load_frags(n, f);
-    int y, x;
-    int idq[2];
-    float4 nv[2][4];
-    idq[0] = pos(0, r, y, x);
-    patch_load(plane, pitch, y, x, nv[0]);
-    idq[1] = pos(1, r, y, x);
-    patch_load(plane, pitch, y, x, nv[1]);
-#pragma unroll
-    for (int g = 0; g < kEmbedGroups; ++g) {
-        if (__ballot(idq[g & 1] >= 0) == 0) break;
-        s4 xf[4];
-        patch_cvt(nv[g & 1], xf);
-        const int cur = idq[g & 1];
-        if (g + 2 < kEmbedGroups) {
-            idq[g & 1] = pos(g + 2, r, y, x);
-            patch_load(plane, pitch, y, x, nv[g & 1]);
-        }
-        float e[2][4];
-        infer16(f, xf, e);
-        store_emb(e, cur, emb, norm);
-    }
+    for (int u = 0; u < 2; ++u) __builtin_memcpy(emb + at * kEmb + 16 * u + 4 * q, &eh[u], sizeof(h4));
+    norm[at] = nrm;
 }
 
 // The lattice embeddings run the network on v_mfma_f32_16x16x32_bf16 (half
@@ -618,7 +545,7 @@ __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __res
     for (int k = 0; k < kEmbedRun; ++k) {
         float e[2][4];
         infer32(f, x, e);
-        store_emb(e, xok && k < nrows ? (yy0 + k) * ew2 + xx : -1, emb, norm);
+        store_emb(e, xok && k < nrows ? (yy0 + k) * ew2 + xx : -1, ew2 * eh2, emb, norm);
         if (k + 1 == kEmbedRun) break;
         // next lattice row: group q takes group q + 1's rows, group 3 the new ones
 #pragma unroll
@@ -645,24 +572,50 @@ __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __res
     }
 }
 
-// Embeddings of the reference blocks (scan order b = by * nx + bx).
+// Embeddings of the reference blocks (scan order b = by * nx + bx), with the
+// same network evaluation as the lattice: each wave embeds kRefGroups groups
+// of 16 blocks, the next group's pixel rows loaded while a group runs (block
+// indices past the end are clamped: computed, not stored).  The first lane of
+// the grid also clears the fallback counter of the re-rank that follows.
+constexpr int kRefGroups = 8;
 __global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restrict__ plane, StageParams p,
-                                                    _Float16* __restrict__ emb, float* __restrict__ norm) {
+                                                    _Float16* __restrict__ emb, float* __restrict__ norm,
+                                                    int* __restrict__ nfallback) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nfallback = 0;
     const int nb = p.nx * p.ny;
     const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
-    const int b0 = wave * 16 * kEmbedGroups;
+    const int b0 = wave * 16 * kRefGroups;
     if (b0 >= nb) return;
-    embed_groups(
-        n, plane, p.pitch,
-        [&](int g, int r, int& y, int& x) {
-            const int b = b0 + 16 * g + r;
-            const int bc = b < nb ? b : nb - 1;
-            const int by = bc / p.nx;
-            y = grid_pos(by, p.block_step, p.height - 8);
-            x = grid_pos(bc - by * p.nx, p.block_step, p.width - 8);
-            return b < nb ? b : -1;
-        },
-        emb, norm);
+    const int lane = lane_id();
+    const int r = lane & 15, q = lane >> 4;
+    auto load = [&](int g, float4 (&v)[4]) {
+        const int b = b0 + 16 * g + r;
+        const int bc = b < nb ? b : nb - 1;
+        const int by = bc / p.nx;
+        const int y = grid_pos(by, p.block_step, p.height - 8);
+        const int x = grid_pos(bc - by * p.nx, p.block_step, p.width - 8);
+        const float* src = plane + (size_t)(y + 2 * q) * p.pitch + x;
+        __builtin_memcpy(&v[0], src, 16);
+        __builtin_memcpy(&v[1], src + 4, 16);
+        __builtin_memcpy(&v[2], src + p.pitch, 16);
+        __builtin_memcpy(&v[3], src + p.pitch + 4, 16);
+    };
+    Frags32 f;
+    load_frags32(n, f);
+    float4 cur[4], nxt[4];
+    load(0, cur);
+#pragma unroll
+    for (int g = 0; g < kRefGroups; ++g) {
+        if (g + 1 < kRefGroups) load(g + 1, nxt);
+        const s8 x[2] = {row_cvt(cur[0], cur[1]), row_cvt(cur[2], cur[3])};
+        float e[2][4];
+        infer32(f, x, e);
+        const int b = b0 + 16 * g + r;
+        store_emb(e, b < nb ? b : -1, nb, emb, norm);
+        if (g + 1 < kRefGroups)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1086,7 +1039,8 @@ __device__ __forceinline__ void rerank_run(const ProposeArgs& a, RerankLds& s) {
             own |= (ok ? 1u : 0u) << ((dy + 1) * 3 + dx + 1);
         }
     // the nine SSDs; patch row pr = plane row cy - 1 + pr = window row cy - t + pr
-    const float* wbase = s.win + (cy - t) * kWinStride + (cx - 2 - L0);
+    // (an 8-byte aligned pair index: cx - 2 - L0 and kWinStride are even)
+    const float2* wbase = reinterpret_cast<const float2*>(s.win) + ((cy - t) * kWinStride + (cx - 2 - L0)) / 2;
     float acc[9][SCALAR ? 1 : 4];
 #pragma unroll
     for (int u = 0; u < 9; ++u)
@@ -1094,7 +1048,7 @@ __device__ __forceinline__ void rerank_run(const ProposeArgs& a, RerankLds& s) {
         for (int c = 0; c < (SCALAR ? 1 : 4); ++c) acc[u][c] = 0.0f;
     auto load_row = [&](int pr, float (&v)[10]) {
         float2 w[6];
-        const float2* src = reinterpret_cast<const float2*>(wbase + pr * kWinStride);
+        const float2* src = wbase + pr * (kWinStride / 2);
 #pragma unroll
         for (int k = 0; k < 6; ++k) w[k] = src[k];
         v[0] = w[0].y;
@@ -1282,11 +1236,10 @@ hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int 
 }
 
 hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
-                             hipStream_t st) {
+                             int* nfallback, hipStream_t st) {
     const int nb = p.nx * p.ny;
-    if (nb <= 0) return hipSuccess;
-    const int waves = (nb + 16 * kEmbedGroups - 1) / (16 * kEmbedGroups);
-    hipLaunchKernelGGL(k_embed_refs, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, p, emb, norm);
+    const int waves = nb > 0 ? (nb + 16 * kRefGroups - 1) / (16 * kRefGroups) : 1;
+    hipLaunchKernelGGL(k_embed_refs, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, p, emb, norm, nfallback);
     return hipGetLastError();
 }
 
