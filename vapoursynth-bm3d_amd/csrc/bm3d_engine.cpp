@@ -649,6 +649,7 @@ int bm3d_profile(bm3d_filter* f, int enable) {
 
 int bm3d_profile_read(const bm3d_filter* f, int idx, const char** name, double* total_ms, int64_t* launches) {
     if (!f) return set_error(BM3D_E_STATE, "bm3d_profile_read: null handle");
+    std::lock_guard<std::mutex> lk(f->mu);
     int k = 0;
     for (int i = 0; i < bm3d_filter::kMaxTimers; ++i) {
         if (!f->prof_name[i]) continue;
