@@ -1310,7 +1310,7 @@ __device__ __forceinline__ void filter8_group(const FilterArgs& a, int g_id, flo
 }
 
 template <int GMAX, bool WIENER, bool ONE, bool SCALAR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_filter8(FilterArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_filter8(FilterArgs a) {
     extern __shared__ float fsm[];
     const int slot = xcd_item(a.nb, 1, 0);
     if (slot < 0) return;
