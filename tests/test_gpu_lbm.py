@@ -148,3 +148,31 @@ def test_training_lowers_loss_and_raises_recall(gpu):
     assert rec["trained"] > rec["untrained"] + 0.02, (rec, fb)
     assert fb["trained"] <= fb["untrained"], (rec, fb)
     assert rec["trained"] > 0.5, (rec, fb)
+
+
+def test_fallback_blocks_get_the_exhaustive_lists(gpu):
+    """A block with fewer than G-1 accepted re-ranked candidates is searched
+    exhaustively again (the cooperative block-list kernel), so every learned
+    list shorter than G must equal the plain matcher's list of that block.  A
+    small th_mse leaves many blocks short of G-1 accepted candidates, and an
+    untrained network makes most of them fall back."""
+    h, w = 96, 128
+    noisy, _ = _plane(h, w, 25.0, 904)
+    kw = dict(profile="fast", sigma=25.0, bm_range=24, group_size=16, th_mse=600.0)
+    plain = _basic(gpu, noisy, **kw)
+    out_p = np.asarray(plain.get_frame(0).planes[0])
+    learned = _basic(gpu, noisy, **kw)
+    learned.filter.set_matcher(lbm.Model(seed=21), 32)
+    learned.get_frame(0)
+    cp, pp, kp = plain.filter.match_lists()
+    cl, pl, kl = learned.filter.match_lists()
+    fb = learned.filter.matcher_fallbacks()
+    short = np.nonzero(cl < 16)[0]
+    print("fallbacks", fb, "short lists", len(short), "of", len(cl), flush=True)
+    assert fb > 0 and len(short) > 0
+    for b in short:
+        n = int(cl[b])
+        assert n == int(cp[b]), b
+        assert np.array_equal(pl[b, :n], pp[b, :n]), b
+        assert np.array_equal(kl[b, :n].view(np.uint32), kp[b, :n].view(np.uint32)), b
+    del out_p
