@@ -491,7 +491,7 @@ __device__ __forceinline__ s8 row_cvt(const float4& a, const float4& b) {
 // further down (kLat = 2 pixel rows) its rows are those group q + 1 held: a
 // 16-lane shift, and group 3 loads the two new rows, issued kEmbedAhead
 // lattice rows ahead.
-constexpr int kEmbedRun = 16;
+constexpr int kEmbedRun = 32;
 constexpr int kEmbedAhead = 4;
 static_assert(kLat == 2, "row reuse between lattice rows assumes a stride of two rows");
 
