@@ -74,12 +74,15 @@ def _exact_key(img, j, i, y, x, mode=O.ARITH_SSE2):
     return np.float32(np.float64(d) * DIST_MUL)
 
 
-@pytest.mark.parametrize("arith", [0, 1])
-def test_learned_lists_are_exact_reranks(gpu, arith):
+@pytest.mark.parametrize("arith,step", [(0, None), (1, None), (0, 3), (0, 5)])
+def test_learned_lists_are_exact_reranks(gpu, arith, step):
+    # block_step 8 (the profile's), 3 and 5: the proposal tiles and re-rank
+    # runs cover blocks whose windows overlap by different amounts
     h, w = 96, 128
     noisy, _ = _plane(h, w, 25.0, 902)
     model = lbm.Model(seed=7)  # untrained: proposals are poor, the re-rank must still be exact
-    node = _basic(gpu, noisy, arith=arith, **KW)
+    kw = dict(KW, block_step=step) if step else KW
+    node = _basic(gpu, noisy, arith=arith, **kw)
     node.filter.set_matcher(model, 32)
     node.get_frame(0)
     cnt, pos, key = node.filter.match_lists()
