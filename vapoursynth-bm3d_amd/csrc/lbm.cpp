@@ -128,9 +128,8 @@ int lbm_match(const bm3d_lbm* m, int topk, LbmWork& w, const float* plane, const
     if ((rc = w.nfallback.ensure(sizeof(int)))) return rc;
     if ((rc = w.props.ensure(L::propose_list_bytes(sp)))) return rc;
     const L::Net n = m->net();
-    HIP_TRY(L::launch_embed(n, plane, sp.pitch, W, H, w.emb.as<_Float16>(), w.norm.as<float>(), st));
-    HIP_TRY(L::launch_embed_refs(n, plane, sp, w.remb.as<_Float16>(), w.rnorm.as<float>(), w.nfallback.as<int>(),
-                                 st));
+    HIP_TRY(L::launch_embed(n, plane, sp, w.emb.as<_Float16>(), w.norm.as<float>(), w.remb.as<_Float16>(),
+                            w.rnorm.as<float>(), w.nfallback.as<int>(), st));
     L::ProposeArgs pa{};
     pa.p = sp;
     pa.plane = plane;

@@ -77,13 +77,11 @@ hipError_t launch_train_adam(const Net& n, const float* grad, float* m, float* v
 hipError_t launch_sync_bf16(const Net& n, hipStream_t st);
 hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int npairs, float* target,
                            hipStream_t st);
-// emb/norm hold ceil((W-7)/2) * ceil((H-7)/2) + 1 entries (the last: spare slot), likewise nx * ny + 1 for
-// launch_embed_refs
-hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
-                        hipStream_t st);
-// (also clears *nfallback for the re-rank that follows)
-hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
-                             int* nfallback, hipStream_t st);
+// The lattice embeddings (emb/norm: ceil((W-7)/2) * ceil((H-7)/2) + 1 entries, the last a spare slot) and
+// the reference blocks' (remb/rnorm: nx * ny + 1 entries) in one launch; also clears *nfallback for the
+// re-rank that follows.
+hipError_t launch_embed(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
+                        _Float16* remb, float* rnorm, int* nfallback, hipStream_t st);
 constexpr int kMaxLearnedGroupSize = 64;
 hipError_t launch_propose(const ProposeArgs& a, hipStream_t st);
 size_t propose_list_bytes(const StageParams& p);

@@ -495,12 +495,12 @@ constexpr int kEmbedRun = 32;
 constexpr int kEmbedAhead = 4;
 static_assert(kLat == 2, "row reuse between lattice rows assumes a stride of two rows");
 
-__global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __restrict__ plane, int pitch, int W, int H,
-                                                       _Float16* __restrict__ emb, float* __restrict__ norm) {
+__device__ __forceinline__ void embed_lattice_wg(int wg, const Net& n, const float* __restrict__ plane, int pitch,
+                                                 int W, int H, _Float16* __restrict__ emb, float* __restrict__ norm) {
     const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
     const int gx = (ew2 + 15) / 16;
     const int runs = (eh2 + kEmbedRun - 1) / kEmbedRun;
-    const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int wave = uniform((int)(wg * 4 + (threadIdx.x >> 6)));
     if (wave >= gx * runs) return;
     const int ry = wave / gx;
     const int lane = lane_id();
@@ -578,12 +578,12 @@ __global__ __launch_bounds__(256) void k_embed_lattice(Net n, const float* __res
 // indices past the end are clamped: computed, not stored).  The first lane of
 // the grid also clears the fallback counter of the re-rank that follows.
 constexpr int kRefGroups = 8;
-__global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restrict__ plane, StageParams p,
-                                                    _Float16* __restrict__ emb, float* __restrict__ norm,
-                                                    int* __restrict__ nfallback) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *nfallback = 0;
+__device__ __forceinline__ void embed_refs_wg(int wg, const Net& n, const float* __restrict__ plane,
+                                              const StageParams& p, _Float16* __restrict__ emb,
+                                              float* __restrict__ norm, int* __restrict__ nfallback) {
+    if (wg == 0 && threadIdx.x == 0) *nfallback = 0;
     const int nb = p.nx * p.ny;
-    const int wave = uniform((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+    const int wave = uniform((int)(wg * 4 + (threadIdx.x >> 6)));
     const int b0 = wave * 16 * kRefGroups;
     if (b0 >= nb) return;
     const int lane = lane_id();
@@ -616,6 +616,18 @@ __global__ __launch_bounds__(256) void k_embed_refs(Net n, const float* __restri
 #pragma unroll
             for (int k = 0; k < 4; ++k) cur[k] = nxt[k];
     }
+}
+
+// Both embedding passes in one launch: workgroups [0, lat_wgs) embed the
+// lattice, the rest the reference blocks (independent work, so the reference
+// pass runs beside the lattice pass's last workgroups instead of after them).
+__global__ __launch_bounds__(256) void k_embed(Net n, const float* __restrict__ plane, StageParams p, int lat_wgs,
+                                               _Float16* __restrict__ emb, float* __restrict__ norm,
+                                               _Float16* __restrict__ remb, float* __restrict__ rnorm,
+                                               int* __restrict__ nfallback) {
+    const int wg = (int)blockIdx.x;
+    if (wg < lat_wgs) embed_lattice_wg(wg, n, plane, p.pitch, p.width, p.height, emb, norm);
+    else embed_refs_wg(wg - lat_wgs, n, plane, p, remb, rnorm, nfallback);
 }
 
 // ---------------------------------------------------------------------------
@@ -1226,20 +1238,15 @@ hipError_t launch_pair_ssd(const float* plane, int pitch, const int4* pairs, int
     return hipGetLastError();
 }
 
-hipError_t launch_embed(const Net& n, const float* plane, int pitch, int W, int H, _Float16* emb, float* norm,
-                        hipStream_t st) {
-    const int ew2 = lat_count(W - 7), eh2 = lat_count(H - 7);
-    if (ew2 <= 0 || eh2 <= 0) return hipSuccess;
-    const int waves = ((ew2 + 15) / 16) * ((eh2 + kEmbedRun - 1) / kEmbedRun);
-    hipLaunchKernelGGL(k_embed_lattice, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, pitch, W, H, emb, norm);
-    return hipGetLastError();
-}
-
-hipError_t launch_embed_refs(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
-                             int* nfallback, hipStream_t st) {
+hipError_t launch_embed(const Net& n, const float* plane, const StageParams& p, _Float16* emb, float* norm,
+                        _Float16* remb, float* rnorm, int* nfallback, hipStream_t st) {
+    const int ew2 = lat_count(p.width - 7), eh2 = lat_count(p.height - 7);
+    const int lat_waves = ew2 > 0 && eh2 > 0 ? ((ew2 + 15) / 16) * ((eh2 + kEmbedRun - 1) / kEmbedRun) : 0;
     const int nb = p.nx * p.ny;
-    const int waves = nb > 0 ? (nb + 16 * kRefGroups - 1) / (16 * kRefGroups) : 1;
-    hipLaunchKernelGGL(k_embed_refs, dim3((waves + 3) / 4), dim3(256), 0, st, n, plane, p, emb, norm, nfallback);
+    const int ref_waves = nb > 0 ? (nb + 16 * kRefGroups - 1) / (16 * kRefGroups) : 1;  // >= 1: clears *nfallback
+    const int lat_wgs = (lat_waves + 3) / 4, ref_wgs = (ref_waves + 3) / 4;
+    hipLaunchKernelGGL(k_embed, dim3(lat_wgs + ref_wgs), dim3(256), 0, st, n, plane, p, lat_wgs, emb, norm, remb, rnorm,
+                       nfallback);
     return hipGetLastError();
 }
 
