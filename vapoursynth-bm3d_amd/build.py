@@ -22,7 +22,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("BM3D_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["bm3d_kernels.hip", "bm3d_vkernels.hip", "lbm_kernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp", "lbm.cpp",
+SOURCES = ["bm3d_kernels.hip", "bm3d_agg.hip", "bm3d_vkernels.hip", "lbm_kernels.hip", "bm3d_engine.cpp", "bm3d_run.cpp", "lbm.cpp",
            "bm3d_batch.cpp",
            "bm3d_noalloc.cpp"]
 

@@ -17,6 +17,30 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Item of consumer `sub` (a wave) of this workgroup in a launch of
+// 8 * ceil(ceil(total / per) / 8) workgroups with `per` consumers each, or -1.
+// Workgroup g runs on XCD g % 8, so the items are split into eight contiguous
+// ranges, one per XCD: neighbouring reference blocks and tiles, which read
+// overlapping pixels and contributions, share an L2.
+__device__ __forceinline__ int xcd_item(int total, int per, int sub) {
+    const int span = (total + 7) >> 3;
+    const int x = (int)(blockIdx.x & 7);
+    const int i = x * span + (int)(blockIdx.x >> 3) * per + sub;
+    return i < min(total, (x + 1) * span) ? i : -1;
+}
+
+// Frame of global item idx: the last f with starts[f] <= idx (starts[0] = 0,
+// ascending, nframes entries; wave-uniform).
+__device__ __forceinline__ int batch_frame(const int* __restrict__ starts, int nframes, int idx) {
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (uniform(starts[mid]) <= idx) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Position k of the reference-block grid along one axis: 0, s, 2s, ... with
 // the last one clamped to dim - B (source/BM3D_Base.cpp:312-345).
 __device__ __forceinline__ int grid_pos(int k, int step, int last) {
@@ -399,6 +423,7 @@ __device__ __forceinline__ void reach_rect(const StageParams& p, int y0, int x0,
 constexpr int kAggRecCap = 128;    // records buffered per wave between walk and consume (small: occupancy)
 constexpr int kAggRecWords = 3;    // position, member index b*G + z, reference block b
 constexpr int kAggBatch = 8;       // records per load batch of the consume phase
+constexpr int kAggWalk = 4;        // walk steps whose loads are in flight together
 
 // One batch of kAggBatch records of the consume phase: lane j (< kAggBatch)
 // holds record j's position and weights, every lane its element of each
@@ -424,33 +449,45 @@ __device__ __forceinline__ void agg_load_batch(const int* rec, int r0, int nrec,
     const int rb_l = rec[idx + 2];
 #pragma unroll
     for (int k = 0; k < NP; ++k) t.w[k] = ld_global(d[k] + rb_l);
+    // the block's base is wave-uniform (scalar registers) and the lane's
+    // element a 32-bit offset, so each load needs no vector address add
+    const unsigned lcu = (unsigned)lc;
 #pragma unroll
     for (int j = 0; j < kAggBatch; ++j) {
-        const size_t off = (size_t)__builtin_amdgcn_readlane(re_l, j) * BB + lc;
+        const size_t blk = (size_t)(unsigned)__builtin_amdgcn_readlane(re_l, j) * (unsigned)BB;
 #pragma unroll
-        for (int k = 0; k < NP; ++k) t.v[k][j] = ld_global(c[k] + off);
+        for (int k = 0; k < NP; ++k) t.v[k][j] = ld_global(c[k] + blk + lcu);
     }
 }
 
-// Adds, in order, the records [r0, r0 + kAggBatch) below nrec.
+// Adds, in order, the records [r0, r0 + kAggBatch) below nrec.  Branch-free:
+// a lane whose element falls outside the tile (or lies past the block, or
+// belongs to a record past nrec) adds into its own slot of the row padding
+// (columns kAggTW.. of row lane / 8), which no output reads, so the records
+// follow one another with no exec-mask changes and the compiler can compute
+// the next record's address while this record's LDS round trip is in flight.
+// The LDS operations of one wave complete in order, so a record's reads see
+// the writes of the records before it.
+static_assert(kAggStride - kAggTW >= 8 && kAggTH >= 8, "row padding holds one slot per lane");
 template <int NP>
 __device__ __forceinline__ void agg_apply_batch(const AggBatch<NP>& t, int r0, int nrec, int BB, int dy, int dx,
                                                 int y0, int x0, float* acc) {
     const int lane = lane_id();
+    const int spare = (lane >> 3) * kAggStride + kAggTW + (lane & 7);
+    const int loff = dy * kAggStride + dx;  // the lane's element relative to the block origin
+    const bool lane_ok = lane < BB;
 #pragma unroll
     for (int j = 0; j < kAggBatch; ++j) {
-        if (r0 + j < nrec) {
-            const int pk = __builtin_amdgcn_readlane(t.pk, j);
-            const int ty = (pk >> 16) + dy - y0, tx = (pk & 0xffff) + dx - x0;
-            if (lane < BB && ty >= 0 && ty < kAggTH && tx >= 0 && tx < kAggTW) {
-                float* an = acc + ty * kAggStride + tx;
+        const int pk = __builtin_amdgcn_readlane(t.pk, j);
+        const int sy = (pk >> 16) - y0, sx = (pk & 0xffff) - x0;  // block origin in the tile (uniform)
+        const bool ok = r0 + j < nrec && lane_ok && (unsigned)(sy + dy) < (unsigned)kAggTH &&
+                        (unsigned)(sx + dx) < (unsigned)kAggTW;
+        float* an = acc + (ok ? sy * kAggStride + sx + loff : spare);
 #pragma unroll
-                for (int k = 0; k < NP; ++k) {
-                    const float w = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t.w[k]), j));
-                    an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + t.v[k][j];
-                    an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + w;
-                }
-            }
+        for (int k = 0; k < NP; ++k) {
+            const float w = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, t.w[k]), j));
+            an[k * 2 * kAggPlane] = an[k * 2 * kAggPlane] + t.v[k][j];
+            an[k * 2 * kAggPlane + kAggPlane] = an[k * 2 * kAggPlane + kAggPlane] + w;
         }
     }
 }
@@ -521,18 +558,18 @@ __device__ __forceinline__ void agg_tile_lds(const StageParams& p, const int* __
         z = z < G ? z : G - 1;
     };
     int nrec = 0;
-    for (int it = 0; it < total; it += 4) {
-        int sb[4], sz[4], sn[4], spk[4], sfz[4];
-        bool slive[4];
+    for (int it = 0; it < total; it += kAggWalk) {
+        int sb[kAggWalk], sz[kAggWalk], sn[kAggWalk], spk[kAggWalk], sfz[kAggWalk];
+        bool slive[kAggWalk];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kAggWalk; ++u) {
             step_ids(it + u, sb[u], sz[u], slive[u]);
             sn[u] = count[sb[u]];
             spk[u] = pos[sb[u] * G + sz[u]];
             sfz[u] = frame ? frame[sb[u] * G + sz[u]] : fr;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kAggWalk; ++u) {
             const int qy = spk[u] >> 16, qx = spk[u] & 0xffff;
             const bool ov = slive[u] && sz[u] < sn[u] && sfz[u] == fr && qy <= y1 && qy + B > y0 && qx <= x1 &&
                             qx + B > x0;

@@ -160,6 +160,13 @@ hipError_t launch_aggregate_batch(const AggArgs* aa, const int* starts, int nfra
                                   const BatchShape& sh, hipStream_t st);
 int device_cus();
 int aggregate_tiles(int W, int H);
+
+// Workgroups of a batched launch (xcd_item): one per `per` items, rounded up
+// to a multiple of 8 so that every XCD gets the same number.
+inline int batch_groups(int items, int per) {
+    const long need = ((long)items + per - 1) / per;
+    return (int)((need + 7) / 8 * 8);
+}
 hipError_t launch_aggregate(const AggArgs& a, hipStream_t st);
 hipError_t launch_to_float(const void* src, int bits, size_t spitch, float* dst, size_t dpitch, int W, int H,
                            float gain, float offset, int clip, float lo, float hi, hipStream_t st);
