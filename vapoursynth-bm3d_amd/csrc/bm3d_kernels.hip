@@ -1209,17 +1209,22 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
             } else {
             const int c = x & 3;
             const int ibase = y * 2 + (x >> 2);
+            // chain stride = 16 (mod 64): the four chains of one store start
+            // in different banks (with stride 16N they would share banks for
+            // N % 4 == 0: a four-way conflict on every store)
+            constexpr int kChain = 16 * N + (80 - (16 * N) % 64) % 64;
+            static_assert(3 * kChain + 16 * N <= (N <= 8 ? 8 : N <= 16 ? 16 : 32) * 72, "chains fit the group buffer");
 #pragma unroll
             for (int k = 0; k < N; ++k) {
                 const float r2 = R[k] * R[k];
                 const float w = r2 * rcp_host(a.rcp_table, a.rcp_shift, r2 + sig2);
                 X[k] = X[k] * w;
-                w2[c * 16 * N + k * 16 + ibase] = w * w;
+                w2[c * kChain + k * 16 + ibase] = w * w;
             }
             wave_sync_lds();
             float part = 0.0f;
             if (lane < 4) {
-                const float4* ch = reinterpret_cast<const float4*>(w2 + lane * 16 * N);
+                const float4* ch = reinterpret_cast<const float4*>(w2 + lane * kChain);
 #pragma unroll
                 for (int q0 = 0; q0 < 4 * N; q0 += 8) {
                     float4 v4[8];
