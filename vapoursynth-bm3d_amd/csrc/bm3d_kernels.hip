@@ -997,25 +997,37 @@ __device__ __forceinline__ void dct8_inv(const float* in, float* out) {
 // for global vector loads on this target).
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
 
-// Gather and forward x pass of the N member blocks of one plane: lane takes
-// the lines L = (z, y), loads the member's row straight from the plane (two
-// 16-byte loads), transforms it in registers and writes the coefficients to
-// g[z][y][x] (row stride 9, so the z pass's column reads hit distinct banks).
 template <int N>
-__device__ __forceinline__ void rows_fwd(float* g, const float* const* planes, const int* zyx, int pitch,
-                                         bool single_frame) {
+struct RowLoads {
+    static constexpr int kIters = (N * 8 + 63) / 64;
+    f4u a[kIters], b[kIters];
+};
+template <int N>
+__device__ __forceinline__ void rows_load(RowLoads<N>& r, const float* base0, const float* const* planes,
+                                          const int* zyx, int pitch) {
     const int lane = lane_id();
     constexpr int NL = N * 8;
 #pragma unroll
-    for (int L0 = 0; L0 < NL; L0 += 64) {
-        const int L = L0 + lane;
+    for (int i = 0; i < RowLoads<N>::kIters; ++i) {
+        const int L = i * 64 + lane;
         if (NL % 64 == 0 || L < NL) {
             const int z = L >> 3, y = L & 7;
-            const float* base = single_frame ? planes[0] : planes[zyx[3 * z]];
+            const float* base = base0 ? base0 : planes[zyx[3 * z]];
             const float* row = base + (size_t)(zyx[3 * z + 1] + y) * pitch + zyx[3 * z + 2];
-            const f4u a = *reinterpret_cast<const f4u*>(row);
-            const f4u b = *reinterpret_cast<const f4u*>(row + 4);
-            const float in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+            r.a[i] = *(const __attribute__((address_space(1))) f4u*)row;
+            r.b[i] = *(const __attribute__((address_space(1))) f4u*)(row + 4);
+        }
+    }
+}
+template <int N>
+__device__ __forceinline__ void rows_fwd(float* g, const RowLoads<N>& r) {
+    const int lane = lane_id();
+    constexpr int NL = N * 8;
+#pragma unroll
+    for (int i = 0; i < RowLoads<N>::kIters; ++i) {
+        const int L = i * 64 + lane;
+        if (NL % 64 == 0 || L < NL) {
+            const float in[8] = {r.a[i].x, r.a[i].y, r.a[i].z, r.a[i].w, r.b[i].x, r.b[i].y, r.b[i].z, r.b[i].w};
             float out[8];
             dct8_fwd(in, out);
             float* dst = g + L * 9;
@@ -1150,18 +1162,59 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
     const int G = a.p.group;
     const int y = lane >> 3, x = lane & 7;
     const int off = y * 9 + x;
+    const bool single = a.frame == nullptr;
+    // Gathers are prefetched: each one's loads are issued while the previous
+    // gather is being transformed (hard threshold: the next plane's source
+    // rows; Wiener: this plane's source rows during the reference transform,
+    // the next plane's reference rows once the Wiener gains are looked up).
+    // Global loads complete in order, so everything else the group reads
+    // (plane bases, thresholds, Wiener sigma, amplitudes) is read up front: a
+    // load issued later would wait for the gather in flight.
+    const bool q0 = a.process[0] != 0, q1 = !ONE && a.process[1] != 0, q2 = !ONE && a.process[2] != 0;
+    if (!q0 && !q1 && !q2) return;
+    const float* sb0 = (single && q0) ? uniform_ptr(a.src[0][0]) : nullptr;
+    const float* sb1 = (single && q1) ? uniform_ptr(a.src[1][0]) : nullptr;
+    const float* sb2 = (single && q2) ? uniform_ptr(a.src[2][0]) : nullptr;
+    const float* rb0 = (WIENER && single && q0) ? uniform_ptr(a.ref[0][0]) : nullptr;
+    const float* rb1 = (WIENER && single && q1) ? uniform_ptr(a.ref[1][0]) : nullptr;
+    const float* rb2 = (WIENER && single && q2) ? uniform_ptr(a.ref[2][0]) : nullptr;
+    auto thr_of = [&](bool q, int pl, int k) { return (!WIENER && q) ? uniformf(a.tables[pl].thr[N - 1][k]) : 0.0f; };
+    const float t00 = thr_of(q0, 0, 0), t01 = thr_of(q0, 0, 1), t02 = thr_of(q0, 0, 2), t03 = thr_of(q0, 0, 3);
+    const float t10 = thr_of(q1, 1, 0), t11 = thr_of(q1, 1, 1), t12 = thr_of(q1, 1, 2), t13 = thr_of(q1, 1, 3);
+    const float t20 = thr_of(q2, 2, 0), t21 = thr_of(q2, 2, 1), t22 = thr_of(q2, 2, 2), t23 = thr_of(q2, 2, 3);
+    const float s20 = (WIENER && q0) ? uniformf(a.tables[0].wiener_sigma_sqr[N - 1]) : 0.0f;
+    const float s21 = (WIENER && q1) ? uniformf(a.tables[1].wiener_sigma_sqr[N - 1]) : 0.0f;
+    const float s22 = (WIENER && q2) ? uniformf(a.tables[2].wiener_sigma_sqr[N - 1]) : 0.0f;
+    const double am0 = q0 ? uniformd(a.tables[0].final_amp[N - 1]) : 1.0;
+    const double am1 = q1 ? uniformd(a.tables[1].final_amp[N - 1]) : 1.0;
+    const double am2 = q2 ? uniformd(a.tables[2].final_amp[N - 1]) : 1.0;
+    RowLoads<N> rl;
+    // static plane indices throughout (a plane index known only at run time
+    // would index the kernel-argument arrays through scratch)
+#define BM3D_LOAD_SRC(P) rows_load<N>(rl, sb##P, a.src[P], zyx, a.p.pitch)
+#define BM3D_LOAD_REF(P) rows_load<N>(rl, rb##P, a.ref[P], zyx, a.p.pitch)
+    if (WIENER) {
+        if (q0) BM3D_LOAD_REF(0); else if (q1) BM3D_LOAD_REF(1); else BM3D_LOAD_REF(2);
+    } else {
+        if (q0) BM3D_LOAD_SRC(0); else if (q1) BM3D_LOAD_SRC(1); else BM3D_LOAD_SRC(2);
+    }
+#pragma unroll
     for (int pl = 0; pl < (ONE ? 1 : 3); ++pl) {
-        if (!pick3(a.process, pl)) continue;
-        const float* const* srcp = pick3(a.src, pl);
-        const float* const* refp = pick3(a.ref, pl);
-        const PlaneTables& tb = a.tables[pl];
+        if (!(pl == 0 ? q0 : pl == 1 ? q1 : q2)) continue;
         float X[N];
         float den_w;
         if (!WIENER) {
-            rows_fwd<N>(gs, srcp, zyx, a.p.pitch, a.frame == nullptr);
+            rows_fwd<N>(gs, rl);
+            if (pl == 0) {
+                if (q1) BM3D_LOAD_SRC(1); else if (q2) BM3D_LOAD_SRC(2);
+            } else if (pl == 1) {
+                if (q2) BM3D_LOAD_SRC(2);
+            }
             float thr4[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) thr4[q] = tb.thr[N - 1][q];
+            thr4[0] = pl == 0 ? t00 : pl == 1 ? t10 : t20;
+            thr4[1] = pl == 0 ? t01 : pl == 1 ? t11 : t21;
+            thr4[2] = pl == 0 ? t02 : pl == 1 ? t12 : t22;
+            thr4[3] = pl == 0 ? t03 : pl == 1 ? t13 : t23;
             pass_y<true>(gs, N);
             zfwdN<N>(gs, off, X);
             int kept = 0;
@@ -1178,16 +1231,17 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
         } else {
             // one LDS buffer: the reference group's transform goes to registers
             // first, then the source group reuses the buffer
-            rows_fwd<N>(gs, refp, zyx, a.p.pitch, a.frame == nullptr);
+            rows_fwd<N>(gs, rl);
+            if (pl == 0) BM3D_LOAD_SRC(0); else if (pl == 1) BM3D_LOAD_SRC(1); else BM3D_LOAD_SRC(2);
             pass_y<true>(gs, N);
             float R[N];
             zfwdN<N>(gs, off, R);
             wave_sync_lds();
-            rows_fwd<N>(gs, srcp, zyx, a.p.pitch, a.frame == nullptr);
+            rows_fwd<N>(gs, rl);
             pass_y<true>(gs, N);
             zfwdN<N>(gs, off, X);
             wave_sync_lds();
-            const float sig2 = tb.wiener_sigma_sqr[N - 1];
+            const float sig2 = pl == 0 ? s20 : pl == 1 ? s21 : s22;
             // w^2 stored chain-major: SSE2 lane c = x % 4 takes the flat
             // indices e = z*64 + y*8 + x in increasing order (BM3D_Final.cpp:80-104)
             float* w2 = gs;
@@ -1248,15 +1302,22 @@ __device__ __forceinline__ void group8(const FilterArgs& a, int g_id, const int*
             l2 = l2 > 1.1920928955078125e-07f ? l2 : 1.1920928955078125e-07f;
             den_w = 1.0f / l2;
             wave_sync_lds();
+            if (pl == 0) {
+                if (q1) BM3D_LOAD_REF(1); else if (q2) BM3D_LOAD_REF(2);
+            } else if (pl == 1) {
+                if (q2) BM3D_LOAD_REF(2);
+            }
         }
         zinvN<N>(gs, off, X);
         wave_sync_lds();
         pass_y<false>(gs, N);
-        const float num_w = (float)((double)den_w / tb.final_amp[N - 1]);
+        const float num_w = (float)((double)den_w / (pl == 0 ? am0 : pl == 1 ? am1 : am2));
         rows_inv_store<N>(gs, pick3(a.contrib, pl) + (size_t)g_id * G * 64, num_w);
         if (lane == 0) pick3(a.denw, pl)[g_id] = den_w;
     }
 }
+#undef BM3D_LOAD_SRC
+#undef BM3D_LOAD_REF
 
 // One wave per group, taken from `order` (groups bucketed by size so that
 // neighbouring waves run the same specialisation).
