@@ -1773,6 +1773,28 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
     if (a.p.block == 8 && gm > 0) {
         const size_t lds = filter8_lds_bytes(a.p.group, a.wiener);
         const bool one = !a.process[1] && !a.process[2];
+        if (!one && a.wiener) {
+            // Wiener on several planes: one launch per plane of the
+            // single-plane kernel, whose waves hold one plane's state (G = 32:
+            // 125 instead of 168 registers, four waves per SIMD instead of
+            // three); each plane's results are those of the joint kernel.  The
+            // hard threshold keeps the joint kernel (six waves per SIMD
+            // already, and its cross-plane gather prefetch).
+            for (int p = 0; p < 3; ++p) {
+                if (!a.process[p]) continue;
+                FilterArgs ap = a;
+                ap.process[0] = 1;
+                ap.process[1] = ap.process[2] = 0;
+                ap.src[0] = a.src[p];
+                ap.ref[0] = a.ref[p];
+                ap.contrib[0] = a.contrib[p];
+                ap.denw[0] = a.denw[p];
+                ap.tables = a.tables + p;
+                const hipError_t e = launch_filter(ap, st);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
         const int grid = (a.nb + 7) / 8 * 8;  // xcd_item: a multiple of 8
         // the hard threshold is the same in both arithmetic flavours
         const bool sc = a.wiener && a.p.arith;
