@@ -1796,7 +1796,8 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
             return hipSuccess;
         }
         const int grid = (a.nb + 7) / 8 * 8;  // xcd_item: a multiple of 8
-        // the hard threshold is the same in both arithmetic flavours
+        // the hard threshold is the same in both arithmetic flavours; Wiener
+        // reaches here with one plane only (split above)
         const bool sc = a.wiener && a.p.arith;
 #define BM3D_F8(GM, W, O, S)                                                             \
     if (gm == GM && a.wiener == W && one == O && sc == S) {                              \
@@ -1806,18 +1807,12 @@ hipError_t launch_filter(const FilterArgs& a, hipStream_t st) {
         BM3D_F8(8, 0, false, false)
         BM3D_F8(16, 0, false, false)
         BM3D_F8(32, 0, false, false)
-        BM3D_F8(8, 1, false, false)
-        BM3D_F8(16, 1, false, false)
-        BM3D_F8(32, 1, false, false)
         BM3D_F8(8, 0, true, false)
         BM3D_F8(16, 0, true, false)
         BM3D_F8(32, 0, true, false)
         BM3D_F8(8, 1, true, false)
         BM3D_F8(16, 1, true, false)
         BM3D_F8(32, 1, true, false)
-        BM3D_F8(8, 1, false, true)
-        BM3D_F8(16, 1, false, true)
-        BM3D_F8(32, 1, false, true)
         BM3D_F8(8, 1, true, true)
         BM3D_F8(16, 1, true, true)
         BM3D_F8(32, 1, true, true)
