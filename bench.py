@@ -480,6 +480,16 @@ def main():
                      "executed_tflops": (round(float(np.mean(executed[dom])) / (stage_ms * 1e-3) / 1e12, 3)
                                          if executed.get(dom) else None)},
         "kernels_ms_per_frame": {k: round(v[0], 4) for k, v in kern.items()},
+        # the matcher against the same FP32 peak and against the ceiling of
+        # its exact-order SSD (separate sub, mul and add: no FMA, so at most
+        # half the FMA-counted peak)
+        "match_roofline": {
+            name: {"achieved": round(float(np.mean(work[name])) / (kern[name][0] * 1e-3) / 1e12, 3),
+                   "unit": "TFLOP/s", "frac": round(float(np.mean(work[name])) / (kern[name][0] * 1e-3) / 1e12
+                                                    / PEAK_FP32_TFLOPS, 4),
+                   "frac_of_non_fma_ceiling": round(float(np.mean(work[name])) / (kern[name][0] * 1e-3) / 1e12
+                                                    / (PEAK_FP32_TFLOPS / 2), 4)}
+            for name in ("k_match(basic)", "k_match(final)") if name in kern and work.get(name)},
     }
     if not args.no_learned:
         print("learned block-matching leg (4K)", file=sys.stderr, flush=True)
