@@ -182,10 +182,11 @@ __device__ inline void rank_sort_selected(uint32_t* sel_key, int* sel_idx, int k
         // one entry per lane
         const uint32_t rk = lane < k ? sel_key[lane] : 0xFFFFFFFFu;
         const int ri = lane < k ? sel_idx[lane] : 0x7FFFFFFF;
+        // the other entries come from the lanes' registers (no LDS round trip per step)
         int rr = 0;
         for (int e2 = 0; e2 < k; ++e2) {
-            const uint32_t k2 = sel_key[e2];
-            const int i2 = sel_idx[e2];
+            const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)rk, e2);
+            const int i2 = __builtin_amdgcn_readlane(ri, e2);
             rr += (k2 < rk) || (k2 == rk && i2 < ri);
         }
         __builtin_amdgcn_wave_barrier();
