@@ -323,15 +323,19 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
             nsel += __popcll(m);
         }
     } else {
-        // bound: t0 = kk-th smallest lane minimum
+        // bound: t0 = the kk-th smallest lane minimum rounded up to a multiple
+        // of 2^16 minus one (a 16-step descent over the high half): at least
+        // kk lane minima are <= t0, so the kk smallest keys all are, and the
+        // exact cut is found among the compacted keys below
         uint32_t lm = 0xFFFFFFFFu;
 #pragma unroll
         for (int q = 0; q < NK; ++q) lm = keys[q] < lm ? keys[q] : lm;
         uint32_t t0 = 0;
-        for (int bit = 31; bit >= 0; --bit) {
+        for (int bit = 31; bit >= 16; --bit) {
             const uint32_t trial = t0 | (1u << bit);
             if (__popcll(__ballot(lm < trial)) < kk) t0 = trial;
         }
+        t0 |= 0xFFFFu;
         int M = 0;
         if (t0 != 0xFFFFFFFFu) {
 #pragma unroll
@@ -351,55 +355,61 @@ __device__ __forceinline__ void match8_block(const float* __restrict__ plane, co
                 base += __popcll(m);
             }
             wave_sync_lds();
-            constexpr int SR = kSelCap / 64;
-            uint32_t ck[SR];
-            int ci[SR];
+            // the compacted pairs, SR per lane: as few registers as M needs
+            auto pick = [&](auto src) {
+                constexpr int SR = decltype(src)::value;
+                uint32_t ck[SR];
+                int ci[SR];
 #pragma unroll
-            for (int u = 0; u < SR; ++u) {
-                const int e = lane + 64 * u;
-                ck[u] = e < M ? s_key[e] : 0xFFFFFFFFu;
-                ci[u] = e < M ? s_idx[e] : 0x7FFFFFFF;
-            }
-            // exact k-th smallest key among the compacted pairs
-            uint32_t tv = 0;
-            for (int bit = 31; bit >= 0; --bit) {
-                const uint32_t trial = tv | (1u << bit);
-                int cnt = 0;
-#pragma unroll
-                for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] < trial));
-                if (cnt < kk) tv = trial;
-            }
-            int less = 0, eq = 0;
-#pragma unroll
-            for (int u = 0; u < SR; ++u) {
-                less += __popcll(__ballot(ck[u] < tv));
-                eq += __popcll(__ballot(ck[u] == tv));
-            }
-            const int need = kk - less;
-            int ithr = 0x7FFFFFFF;
-            if (eq > need) {
-                // equal keys at the cut: the earliest candidates win
-                int it = 0;
-                for (int bit = 20; bit >= 0; --bit) {
-                    const int trial = it | (1 << bit);
+                for (int u = 0; u < SR; ++u) {
+                    const int e = lane + 64 * u;
+                    ck[u] = e < M ? s_key[e] : 0xFFFFFFFFu;
+                    ci[u] = e < M ? s_idx[e] : 0x7FFFFFFF;
+                }
+                // exact k-th smallest key among the compacted pairs
+                uint32_t tv = 0;
+                for (int bit = 31; bit >= 0; --bit) {
+                    const uint32_t trial = tv | (1u << bit);
                     int cnt = 0;
 #pragma unroll
-                    for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] == tv && ci[u] < trial));
-                    if (cnt < need) it = trial;
+                    for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] < trial));
+                    if (cnt < kk) tv = trial;
                 }
-                ithr = it + 1;
-            }
+                int less = 0, eq = 0;
 #pragma unroll
-            for (int u = 0; u < SR; ++u) {
-                const bool sel = ck[u] < tv || (ck[u] == tv && ci[u] < ithr);
-                const uint64_t m = __ballot(sel);
-                const int at = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-                if (sel) {
-                    sel_key[at] = ck[u];
-                    sel_idx[at] = ci[u];
+                for (int u = 0; u < SR; ++u) {
+                    less += __popcll(__ballot(ck[u] < tv));
+                    eq += __popcll(__ballot(ck[u] == tv));
                 }
-                nsel += __popcll(m);
-            }
+                const int need = kk - less;
+                int ithr = 0x7FFFFFFF;
+                if (eq > need) {
+                    // equal keys at the cut: the earliest candidates win
+                    int it = 0;
+                    for (int bit = 20; bit >= 0; --bit) {
+                        const int trial = it | (1 << bit);
+                        int cnt = 0;
+#pragma unroll
+                        for (int u = 0; u < SR; ++u) cnt += __popcll(__ballot(ck[u] == tv && ci[u] < trial));
+                        if (cnt < need) it = trial;
+                    }
+                    ithr = it + 1;
+                }
+#pragma unroll
+                for (int u = 0; u < SR; ++u) {
+                    const bool sel = ck[u] < tv || (ck[u] == tv && ci[u] < ithr);
+                    const uint64_t m = __ballot(sel);
+                    const int at = nsel + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+                    if (sel) {
+                        sel_key[at] = ck[u];
+                        sel_idx[at] = ci[u];
+                    }
+                    nsel += __popcll(m);
+                }
+            };
+            if (M <= 64) pick(std::integral_constant<int, 1>{});
+            else if (M <= 128) pick(std::integral_constant<int, 2>{});
+            else pick(std::integral_constant<int, kSelCap / 64>{});
         } else {
             // descent over every key in registers
             uint32_t tv = 0;
