@@ -1167,8 +1167,22 @@ __device__ __forceinline__ void rerank_run(const ProposeArgs& a, RerankLds& s) {
         for (int e1 = sl; e1 < ncomp; e1 += 32) {
             const uint32_t k1 = ck[e1];
             const int w1 = ci[e1];
+            // four entries' LDS reads in flight per step (one round trip per
+            // entry otherwise)
             int rank = 0;
-            for (int e2 = 0; e2 < ncomp; ++e2) {
+            int e2 = 0;
+            for (; e2 + 4 <= ncomp; e2 += 4) {
+                uint32_t k2[4];
+                int w2[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    k2[q] = ck[e2 + q];
+                    w2[q] = ci[e2 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) rank += (k2[q] < k1) || (k2[q] == k1 && w2[q] < w1);
+            }
+            for (; e2 < ncomp; ++e2) {
                 const uint32_t k2 = ck[e2];
                 rank += (k2 < k1) || (k2 == k1 && ci[e2] < w1);
             }
